@@ -1,0 +1,241 @@
+/*
+ * rsh_scatter.h -- C ABI of the MI355X scattering-integral library
+ * (librsh_scatter.so, built from relativistic_solver_hllc_amd/csrc/).
+ *
+ * The library evaluates the radiation-transport scattering integral of
+ * Lartsev-Artem/relativistic_solver_hllc,
+ *
+ *     S[k][c] = sum_j P[k][j] * Ibar[c][j],   Ibar[c][j] = (1/F) sum_f I[j][c][f]
+ *
+ * with P[k][j] = p(w_k . w_j) * w_j / F (reference convention, p = Thomson
+ * 3(1+mu^2)/4, cuda/cuda_utils.cu:7-16) or the FP64 row-normalised form
+ * P[k][j] = p w_j / sum_j p w_j (Henyey-Greenstein or Thomson).
+ *
+ * Reference paths below are relative to solver_hllc_illum_cuda/.
+ *
+ * Conventions (all entry points):
+ *   - every function returns an int status (rsh_status); 0 = success.
+ *     rsh_last_error() returns a thread-local message for the last failure.
+ *   - sizes and offsets are int64_t (the reference's int offsets overflow at
+ *     1.57M cells x 400 directions x 4 faces, e.g. cuda/cuda_kernel.cu:40).
+ *   - data layouts are the reference's: face intensities illum[j][c][f]
+ *     (direction-major, then cell, then face; cuda/cuda_kernel.cu:40-43,
+ *     solve_module/illum/Illum_async.cpp:217) and the scattering integral
+ *     S[k - k_begin][c] (direction-major; cuda/cuda_kernel.cu:34).
+ *   - "stream" arguments are hipStream_t values passed as void*; the
+ *     rsh_problem API instead takes the reference's stream ids (rsh_stream_id).
+ *   - no function in this header allocates host memory the caller must free.
+ */
+#ifndef RSH_SCATTER_H
+#define RSH_SCATTER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSH_ABI_VERSION 1
+
+typedef enum {
+    RSH_OK = 0,
+    RSH_E_ARG = 1,         /* invalid argument (sizes, ranges, null pointers)   */
+    RSH_E_HIP = 2,         /* HIP runtime error                                  */
+    RSH_E_NOMEM = 3,       /* device or pinned host allocation failed            */
+    RSH_E_STATE = 4,       /* call out of order (e.g. split state not packed)    */
+    RSH_E_IO = 5,          /* file could not be opened or parsed                 */
+    RSH_E_UNSUPPORTED = 6  /* configuration outside what the kernels implement   */
+} rsh_status;
+
+typedef enum { RSH_PHASE_THOMSON = 0, RSH_PHASE_HG = 1 } rsh_phase_kind;
+
+/* Arithmetic form of the contraction. */
+typedef enum {
+    RSH_FORM_FP64 = 0,     /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 in/out     */
+    RSH_FORM_BF16X2 = 1,   /* hi+lo BF16 operands, FP32 MFMA accumulation        */
+    RSH_FORM_FP16X2 = 2,   /* hi+lo FP16 operands (power-of-two scaled), FP32 acc */
+    RSH_FORM_BF16X3 = 3    /* hi+mid+lo BF16 operands, FP32 accumulation         */
+} rsh_form;
+
+/* FP64 kernel family. */
+typedef enum {
+    RSH_F64_AUTO = 0,      /* MFMA GEMM with the tuned tile                      */
+    RSH_F64_DIRECT = 1,    /* one thread per (cell, direction), reference order  */
+    RSH_F64_MFMA = 2       /* MFMA GEMM, tile given by rsh_tile                  */
+} rsh_f64_kernel;
+
+/* Stream ids of the reference (solve_module/solve_global_struct.h:234-240). */
+typedef enum {
+    RSH_STREAM_SCATTERING_1 = 0,
+    RSH_STREAM_SCATTERING_2 = 1,
+    RSH_STREAM_PARAMS = 2,
+    RSH_STREAM_COUNT = 3
+} rsh_stream_id;
+
+/* Tile of the FP64 MFMA kernel: waves per workgroup (16 cells per wave) and
+ * j-steps (4 input directions each) per LDS stage of the phase panel. */
+typedef struct {
+    int32_t waves;        /* 4 or 8          */
+    int32_t chunk_steps;  /* 2 or 4          */
+} rsh_tile;
+
+int rsh_abi_version(void);
+const char* rsh_last_error(void);
+
+/* ---------------------------------------------------------------- host only */
+
+/* Fibonacci-sphere direction set: xyz[m][3], w[m] = 4 pi / m, *full_area = sum w.
+ * No reference counterpart (the reference reads directions from a file). */
+int rsh_fibonacci_directions(int32_t m, double* xyz, double* w, double* full_area);
+
+/* Direction text file: "M", M lines "area x y z", "full_area".
+ * Replaces ReadSphereDirectionDecartToSpherical (file_module/reader_txt.cpp:25-47).
+ * cap = capacity of xyz (3*cap doubles) and w; RSH_E_ARG if the file holds more. */
+int rsh_read_directions(const char* path, int32_t cap, int32_t* m, double* xyz, double* w,
+                        double* full_area);
+/* Writes the same format with round-trip (17 significant digit) precision. */
+int rsh_write_directions(const char* path, int32_t m, const double* xyz, const double* w,
+                         double full_area);
+
+/* Block partition of n directions over np ranks.
+ * Replaces GetSend / GetDisp (solve_module/solve_utils.cpp:177-202). */
+int rsh_partition(int32_t np, int32_t n, int32_t* send, int32_t* disp);
+
+/* ---------------------------------------------------------------- device */
+
+int rsh_device_count(int32_t* n);
+
+/* Select the device for the calling thread.
+ * Replaces SetDevice (cuda/cuda_solve.cu:46-67) minus stream creation, which
+ * rsh_create does per problem. */
+int rsh_set_device(int32_t device);
+
+/* Direction set resident on one device: directions, weights and the FP64
+ * phase matrix P[M][M], built on the device once (phase-matrix build kernel).
+ * Replaces grid_directions_init_copy_host_to_device (cuda/cuda_struct.cu:17-38)
+ * together with the per-thread Gamma evaluation of cuda/cuda_kernel.cu:45,106.
+ *   normalize = 0 : P = p w_j / full_area   (reference convention)
+ *   normalize = 1 : P = p w_j / sum_j p w_j (FP64 row normalisation)       */
+typedef struct rsh_dirset_s* rsh_dirset;
+int rsh_dirset_create(int32_t device, int32_t m, const double* xyz, const double* w, double full_area,
+                      int32_t phase, double g, int32_t normalize, rsh_dirset* out);
+int rsh_dirset_destroy(rsh_dirset ds);
+/* Copies the FP64 phase matrix rows [k0,k1) (row-major, M columns) to host. */
+int rsh_dirset_phase_matrix(rsh_dirset ds, int32_t k0, int32_t k1, double* host_out);
+
+/* FP64 scattering integral on caller-owned device memory.
+ *   d_illum : [M][n_cells][faces] doubles (faces = 4 as in the reference, or 1
+ *             for cell-averaged input)
+ *   d_out   : rows k0..k1-1 written to d_out[(k - k0) * ld_out + c]
+ * kernel = rsh_f64_kernel; tile may be NULL (tuned / default tile).
+ * Replaces d_GetS / d_GetS_MPI / d_GetS_MPI_Stream (cuda/cuda_kernel.cu:23-111). */
+int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells,
+                        int32_t k0, int32_t k1, double* d_out, int64_t ld_out, void* stream,
+                        int32_t kernel, const rsh_tile* tile);
+
+/* Seeded log-normal fill exp(mu + sigma * N(0,1)) on the device (counter-based
+ * generator: the value at index i depends only on (seed, i)). */
+int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed,
+                           void* stream);
+
+/* Checks the lane layout of v_mfma_f64_16x16x4f64 on the device against a
+ * host product; *mismatches = number of differing outputs (0 expected). */
+int rsh_selftest_mfma_f64(int32_t device, int32_t* mismatches);
+
+/* ------------------------------------------------ device memory utilities */
+/* Allocation, copies, streams and events through the library's own HIP
+ * runtime, so host programs (and the Python mirror) need no other GPU stack.
+ * No reference counterpart beyond the CUDA runtime calls the reference makes
+ * inline (cuda/cuda_memory.cuh:5-18).
+ * memcpy kind: 0 = host->device, 1 = device->host, 2 = device->device;
+ * stream NULL makes the copy synchronous. */
+int rsh_dev_alloc(int32_t device, int64_t bytes, void** out);
+int rsh_dev_free(void* ptr);
+int rsh_host_alloc_pinned(int64_t bytes, void** out);
+int rsh_host_free_pinned(void* ptr);
+int rsh_dev_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream);
+int rsh_dev_memcpy2d(void* dst, int64_t dpitch, const void* src, int64_t spitch, int64_t width,
+                     int64_t height, int32_t kind);
+int rsh_dev_fill_f64(double* d, int64_t count, double value, void* stream);
+int rsh_dev_scale_f64(double* d, int64_t count, double factor, void* stream);
+int rsh_dev_synchronize(int32_t device);
+int rsh_dev_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
+int rsh_stream_create(int32_t device, void** out);
+int rsh_stream_destroy(void* stream);
+int rsh_stream_synchronize(void* stream);
+int rsh_event_create(void** out);
+int rsh_event_destroy(void* ev);
+int rsh_event_record(void* ev, void* stream);
+int rsh_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
+
+/* -------------------------------------------- reference-shaped problem API */
+
+/* One problem = one mesh (n_cells, faces per cell) + one direction set + the
+ * local output range [k_begin, k_end) of this rank.
+ * rsh_create replaces InitDevice (cuda/cuda_struct.cu:134-178,187-190): it
+ * uploads directions, builds the phase matrix, allocates device illum
+ * [faces*M*N] and scattering [(k_end-k_begin)*N] and creates the three streams
+ * (default priority). Host buffers stay caller-owned (the C++ shim in
+ * rsh_cuda_solve.hpp allocates the pinned grid.Illum / grid.scattering). */
+typedef struct {
+    int32_t device;
+    int64_t n_cells;
+    int32_t faces;
+    int32_t n_dirs;
+    const double* dirs_xyz;
+    const double* weights;
+    double full_area;
+    int32_t k_begin, k_end;
+    int32_t phase;
+    double g;
+    int32_t normalize;
+} rsh_problem_desc;
+
+typedef struct rsh_problem_s* rsh_problem;
+
+int rsh_create(const rsh_problem_desc* desc, rsh_problem* out);
+
+/* Replaces ClearDevice (cuda/cuda_struct.cu:191-195). */
+int rsh_destroy(rsh_problem p);
+
+/* Asynchronous host->device copy of count doubles from host + src_off to
+ * device illum + dst_off on the given stream id.  Replaces both overloads of
+ * CudaSendIllumAsync (cuda/cuda_solve.cu:69-83; the reference always uses
+ * stream 0). The copy is ordered before every later rsh_scatter on any
+ * stream of the problem (the reference leaves this unordered across streams). */
+int rsh_upload_illum(rsh_problem p, const double* host, int64_t count, int64_t dst_off,
+                     int64_t src_off, int32_t stream);
+
+/* Synchronous copy of count doubles to device illum at offset 0.
+ * Replaces CopyIllumOnDevice (cuda/cuda_solve.cu:85-88). */
+int rsh_copy_illum_sync(rsh_problem p, const double* host, int64_t count);
+
+/* Scattering integral for output directions [k0, k1) (within the problem's
+ * [k_begin, k_end)) on the given stream id; when host_out is not NULL the rows
+ * are copied asynchronously to host_out + (k0 - k_begin) * n_cells.
+ * Replaces CalculateIntScatteringAsyncMPIStream / ...AsyncMPI / ...Async /
+ * CalculateIntScattering (cuda/cuda_solve.cu:89-170). The reference's copy
+ * offset "+start" (cuda_solve.cu:107) is corrected to "+start*N". */
+int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* host_out);
+
+/* Waits for a stream id; stream < 0 waits for the whole device.
+ * Replaces CudaSyncStream and CudaWait (cuda/cuda_solve.cu:37-45). */
+int rsh_sync(rsh_problem p, int32_t stream);
+
+/* Device buffers of the problem (for zero-copy use by callers). */
+int rsh_problem_buffers(rsh_problem p, double** d_illum, double** d_scattering);
+
+/* hipStream_t of a stream id. */
+int rsh_problem_stream(rsh_problem p, int32_t stream, void** hip_stream);
+
+/* Milliseconds of the last rsh_scatter kernel on each stream id (hipEvents).
+ * Replaces CudaEventPrint (cuda/cuda_solve.cu:16-36), whose events the
+ * reference never creates. ms must hold RSH_STREAM_COUNT values; streams
+ * without a timed launch report -1. */
+int rsh_event_times(rsh_problem p, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSH_SCATTER_H */

@@ -1,0 +1,151 @@
+"""ctypes wrapper of the CPU oracle (oracle/scatter_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never
+as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VARIANTS = {"parity": "liborc.so", "o3": "liborc_o3.so", "ofast": "liborc_ofast.so"}
+
+i32, i64, f64, vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
+_PROTOS = {
+    "orc_thomson_gamma": (f64, [f64]),
+    "orc_hg_gamma": (f64, [f64, f64]),
+    "orc_phase": (f64, [C.c_int, f64, f64]),
+    "orc_scatter_reference": (None, [vp, i64, i32, i32, vp, vp, f64, i32, i32, i64, i64, vp, i32]),
+    "orc_face_average": (None, [vp, i64, i32, i32, vp]),
+    "orc_phase_matrix": (None, [i32, vp, vp, f64, i32, f64, i32, i32, i32, vp]),
+    "orc_scatter_matrix": (None, [vp, i64, i32, i32, vp, i32, i64, i64, vp, i32]),
+    "orc_get_send": (None, [i32, i32, vp]),
+    "orc_get_disp": (None, [i32, i32, vp]),
+    "orc_recalc_norm": (f64, [vp, vp, i64]),
+    "orc_source_iteration": (None, [vp, vp, i64, i32, vp, f64, i32, vp, i32]),
+    "orc_fibonacci": (None, [i32, vp, vp, C.POINTER(f64)]),
+    "orc_read_directions": (C.c_int, [C.c_char_p, i32, C.POINTER(i32), vp, vp, C.POINTER(f64)]),
+    "orc_max_threads": (C.c_int, []),
+}
+_cache = {}
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load(variant: str = "parity"):
+    if variant in _cache:
+        return _cache[variant]
+    path = os.path.join(HERE, VARIANTS[variant])
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    _cache[variant] = lib
+    return lib
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def scatter_reference(illum, xyz, w, F, k_disp=0, k_count=None, cell_begin=0, cell_end=None, threads=0,
+                      variant="parity", out=None):
+    """GetS + CalculateIntCPU restatement. illum [M][N][faces] -> S [k_count][N]."""
+    lib = load(variant)
+    illum = _c(illum)
+    M, N, faces = illum.shape
+    k_count = M - k_disp if k_count is None else k_count
+    cell_end = N if cell_end is None else cell_end
+    S = np.zeros((k_count, N)) if out is None else out
+    xyz, w = _c(xyz), _c(w)
+    lib.orc_scatter_reference(illum.ctypes.data, N, M, faces, xyz.ctypes.data, w.ctypes.data, float(F), k_disp,
+                              k_count, cell_begin, cell_end, S.ctypes.data, threads)
+    return S
+
+
+def phase_matrix(xyz, w, F, kind=0, g=0.0, normalize=False, k0=0, k1=None):
+    lib = load()
+    xyz, w = _c(xyz), _c(w)
+    M = len(w)
+    k1 = M if k1 is None else k1
+    P = np.empty((k1 - k0, M))
+    lib.orc_phase_matrix(M, xyz.ctypes.data, w.ctypes.data, float(F), kind, float(g), int(normalize), k0, k1,
+                         P.ctypes.data)
+    return P
+
+
+def scatter_matrix(illum, P, cell_begin=0, cell_end=None, threads=0):
+    """S[k][c] = sum_j P[k][j] * mean_f illum[j][c][f]; illum [M][N][faces]."""
+    lib = load()
+    illum, P = _c(illum), _c(P)
+    M, N, faces = illum.shape
+    cell_end = N if cell_end is None else cell_end
+    S = np.zeros((P.shape[0], N))
+    lib.orc_scatter_matrix(illum.ctypes.data, N, M, faces, P.ctypes.data, P.shape[0], cell_begin, cell_end,
+                           S.ctypes.data, threads)
+    return S
+
+
+def get_send(np_, n):
+    out = np.empty(np_, dtype=np.int32)
+    load().orc_get_send(np_, n, out.ctypes.data)
+    return out
+
+
+def get_disp(np_, n):
+    out = np.empty(np_, dtype=np.int32)
+    load().orc_get_disp(np_, n, out.ctypes.data)
+    return out
+
+
+def recalc_norm(old, new):
+    old = _c(old).copy()
+    new = _c(new)
+    r = load().orc_recalc_norm(old.ctypes.data, new.ctypes.data, old.size)
+    return r, old
+
+
+def source_iteration(I0, B, P, albedo, n_iter, threads=0):
+    I = _c(I0).copy()
+    B, P = _c(B), _c(P)
+    M, N = I.shape
+    resid = np.empty(n_iter)
+    load().orc_source_iteration(I.ctypes.data, B.ctypes.data, N, M, P.ctypes.data, float(albedo), n_iter,
+                                resid.ctypes.data, threads)
+    return I, resid
+
+
+def fibonacci(m):
+    xyz = np.empty((m, 3))
+    w = np.empty(m)
+    F = C.c_double()
+    load().orc_fibonacci(m, xyz.ctypes.data, w.ctypes.data, C.byref(F))
+    return xyz, w, F.value
+
+
+def read_directions(path, cap=1 << 16):
+    xyz = np.empty((cap, 3))
+    w = np.empty(cap)
+    m = C.c_int32()
+    F = C.c_double()
+    rc = load().orc_read_directions(path.encode(), cap, C.byref(m), xyz.ctypes.data, w.ctypes.data, C.byref(F))
+    if rc:
+        raise IOError(f"orc_read_directions({path}) -> {rc}")
+    return xyz[: m.value].copy(), w[: m.value].copy(), F.value
+
+
+def gamma(mu):
+    return load().orc_thomson_gamma(float(mu))
+
+
+def phase(kind, g, mu):
+    return load().orc_phase(int(kind), float(g), float(mu))

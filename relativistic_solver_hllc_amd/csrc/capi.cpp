@@ -1,0 +1,654 @@
+// C-ABI layer of librsh_scatter.so (declarations in include/rsh_scatter.h).
+//
+// Owns device state: direction sets (directions, weights, FP64 phase matrix,
+// cached MFMA panels) and problems (device illum / scattering buffers, the
+// three streams of the reference, upload-ordering events, timing events).
+// Every entry point returns an rsh_status; HIP failures become RSH_E_HIP with
+// the HIP error string in rsh_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+    set_error(msg);
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    set_error(m);
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? RSH_E_NOMEM : RSH_E_HIP;
+}
+
+}  // namespace rsh
+
+using namespace rsh;
+
+// ------------------------------------------------------------------ objects
+
+struct PanelEntry {
+    double* d_panel = nullptr;
+    F64Plan plan{};
+    hipEvent_t ready = nullptr;  // recorded after the panel build
+};
+
+struct rsh_dirset_s {
+    int device = 0;
+    int m = 0;
+    double full_area = 0;
+    int phase = 0;
+    double g = 0;
+    int normalize = 0;
+    std::vector<double> xyz, w;
+    double* d_dirs = nullptr;
+    double* d_w = nullptr;
+    double* d_P = nullptr;  // [m][m] FP64 phase matrix
+    std::mutex mu;
+    // key: (k0, k1, faces)
+    std::map<std::tuple<int, int, int>, PanelEntry> panels;
+    bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
+};
+
+struct rsh_problem_s {
+    int device = 0;
+    int64_t n = 0;
+    int faces = 4;
+    int k_begin = 0, k_end = 0;
+    rsh_dirset ds = nullptr;
+    double* d_illum = nullptr;
+    double* d_S = nullptr;
+    hipStream_t streams[RSH_STREAM_COUNT] = {};
+    hipEvent_t upload_ev[RSH_STREAM_COUNT] = {};
+    bool upload_pending[RSH_STREAM_COUNT] = {};
+    hipEvent_t t0[RSH_STREAM_COUNT] = {}, t1[RSH_STREAM_COUNT] = {};
+    bool timed[RSH_STREAM_COUNT] = {};
+    std::mutex mu;
+};
+
+namespace {
+
+int check_device(int dev) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (dev < 0 || dev >= n) return fail(RSH_E_ARG, "device index " + std::to_string(dev) + " out of range [0," +
+                                                       std::to_string(n) + ")");
+    return RSH_OK;
+}
+
+// Panel for rows [k0,k1) and the given face count, built on stream st if absent.
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, hipStream_t st, const PanelEntry** out) {
+    std::lock_guard<std::mutex> lock(ds->mu);
+    auto key = std::make_tuple(k0, k1, faces);
+    auto it = ds->panels.find(key);
+    if (it != ds->panels.end()) {
+        // Make st wait for the build only while it may still be running.
+        if (hipEventQuery(it->second.ready) == hipErrorNotReady)
+            RSH_HIP_TRY(hipStreamWaitEvent(st, it->second.ready, 0));
+        *out = &it->second;
+        return RSH_OK;
+    }
+    if (ds->panels.size() >= 64) {
+        // Bounded cache: drop everything after the device is idle.
+        RSH_HIP_TRY(hipDeviceSynchronize());
+        for (auto& kv : ds->panels) {
+            hipFree(kv.second.d_panel);
+            hipEventDestroy(kv.second.ready);
+        }
+        ds->panels.clear();
+    }
+    PanelEntry pe;
+    pe.plan = plan_f64(ds->m, k1 - k0);
+    const int64_t nd = panel_f64_doubles(pe.plan);
+    RSH_HIP_TRY(hipMalloc(&pe.d_panel, std::max<int64_t>(nd, 1) * sizeof(double)));
+    hipError_t e = launch_panel_f64(ds->d_P, ds->m, k0, k1, 1.0 / faces, pe.plan, pe.d_panel, st);
+    if (e != hipSuccess) {
+        hipFree(pe.d_panel);
+        return hip_fail(e, "panel_f64_kernel");
+    }
+    RSH_HIP_TRY(hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming));
+    RSH_HIP_TRY(hipEventRecord(pe.ready, st));
+    auto res = ds->panels.emplace(key, pe);
+    *out = &res.first->second;
+    return RSH_OK;
+}
+
+const rsh_tile kDefaultTile = {8, 2};
+
+int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int k0, int k1, double* d_out,
+                int64_t ld, hipStream_t st, int kernel, const rsh_tile* tile) {
+    if (!ds) return fail(RSH_E_ARG, "null direction set");
+    if (faces != 1 && faces != 4) return fail(RSH_E_UNSUPPORTED, "faces must be 1 or 4");
+    if (n < 0 || k0 < 0 || k1 > ds->m || k0 > k1) return fail(RSH_E_ARG, "invalid cell count or direction range");
+    if (n == 0 || k0 == k1) return RSH_OK;
+    if (!d_illum || !d_out) return fail(RSH_E_ARG, "null device pointer");
+    if (ld < n) return fail(RSH_E_ARG, "ld_out smaller than n_cells");
+    RSH_HIP_TRY(hipSetDevice(ds->device));
+    if (kernel == RSH_F64_DIRECT) {
+        RSH_HIP_TRY(launch_direct_f64(d_illum, faces, n, ds->m, ds->d_dirs, ds->d_w, ds->full_area, ds->d_P,
+                                      ds->ref_order() ? 1 : 0, k0, k1, d_out, ld, st));
+        return RSH_OK;
+    }
+    if (kernel != RSH_F64_AUTO && kernel != RSH_F64_MFMA) return fail(RSH_E_ARG, "unknown FP64 kernel id");
+    rsh_tile t = (tile && kernel == RSH_F64_MFMA) ? *tile : kDefaultTile;
+    if (!mfma_f64_tile_supported(t.waves, t.chunk_steps))
+        return fail(RSH_E_UNSUPPORTED, "unsupported MFMA tile (waves, chunk_steps)");
+    const PanelEntry* pe = nullptr;
+    int rc = get_panel(ds, k0, k1, faces, st, &pe);
+    if (rc) return rc;
+    RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ds->m, pe->d_panel, pe->plan, k1 - k0, d_out, ld, t.waves,
+                                t.chunk_steps, st));
+    return RSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsh_abi_version(void) { return RSH_ABI_VERSION; }
+
+const char* rsh_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ host only
+
+int rsh_fibonacci_directions(int32_t m, double* xyz, double* w, double* full_area) {
+    if (m <= 0 || !xyz || !w || !full_area) return fail(RSH_E_ARG, "invalid Fibonacci arguments");
+    const double pi = 3.14159265358979323846;
+    const double ga = pi * (3.0 - std::sqrt(5.0));
+    double F = 0;
+    for (int32_t i = 0; i < m; i++) {
+        const double z = 1.0 - (2.0 * i + 1.0) / m;
+        const double r = std::sqrt(std::fmax(0.0, 1.0 - z * z));
+        const double ph = ga * i;
+        xyz[3 * i] = r * std::cos(ph);
+        xyz[3 * i + 1] = r * std::sin(ph);
+        xyz[3 * i + 2] = z;
+        w[i] = 4.0 * pi / m;
+        F += w[i];
+    }
+    *full_area = F;
+    return RSH_OK;
+}
+
+int rsh_read_directions(const char* path, int32_t cap, int32_t* m, double* xyz, double* w, double* full_area) {
+    if (!path || !m || !full_area) return fail(RSH_E_ARG, "null argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(RSH_E_IO, std::string("cannot open direction file ") + path);
+    long n = -1;
+    int rc = RSH_OK;
+    if (std::fscanf(f, "%ld", &n) != 1 || n < 0) rc = fail(RSH_E_IO, "bad direction count");
+    else if (n > cap || (n > 0 && (!xyz || !w))) rc = fail(RSH_E_ARG, "direction file holds more than cap entries");
+    for (long i = 0; rc == RSH_OK && i < n; i++)
+        if (std::fscanf(f, "%lf %lf %lf %lf", &w[i], &xyz[3 * i], &xyz[3 * i + 1], &xyz[3 * i + 2]) != 4)
+            rc = fail(RSH_E_IO, "truncated direction line " + std::to_string(i));
+    if (rc == RSH_OK && std::fscanf(f, "%lf", full_area) != 1) rc = fail(RSH_E_IO, "missing full_area");
+    std::fclose(f);
+    if (rc == RSH_OK) *m = (int32_t)n;
+    return rc;
+}
+
+int rsh_write_directions(const char* path, int32_t m, const double* xyz, const double* w, double full_area) {
+    if (!path || m < 0 || (m > 0 && (!xyz || !w))) return fail(RSH_E_ARG, "invalid arguments");
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(RSH_E_IO, std::string("cannot create ") + path);
+    std::fprintf(f, "%d\n", m);
+    for (int32_t i = 0; i < m; i++)
+        std::fprintf(f, "%.17g %.17g %.17g %.17g\n", w[i], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    std::fprintf(f, "%.17g\n", full_area);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? RSH_OK : fail(RSH_E_IO, "write failed");
+}
+
+int rsh_partition(int32_t np, int32_t n, int32_t* send, int32_t* disp) {
+    if (np <= 0 || n < 0 || !send || !disp) return fail(RSH_E_ARG, "invalid partition arguments");
+    const int32_t q = n / np, r = n % np;
+    int32_t off = 0;
+    for (int32_t i = 0; i < np; i++) {
+        send[i] = q + (i < r ? 1 : 0);
+        disp[i] = off;
+        off += send[i];
+    }
+    return RSH_OK;
+}
+
+// ------------------------------------------------------------------ device
+
+int rsh_device_count(int32_t* n) {
+    if (!n) return fail(RSH_E_ARG, "null argument");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e == hipErrorNoDevice) c = 0;
+    else if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    *n = c;
+    return RSH_OK;
+}
+
+int rsh_set_device(int32_t device) {
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    return RSH_OK;
+}
+
+int rsh_dirset_create(int32_t device, int32_t m, const double* xyz, const double* w, double full_area,
+                      int32_t phase, double g, int32_t normalize, rsh_dirset* out) {
+    if (!out || m <= 0 || !xyz || !w) return fail(RSH_E_ARG, "invalid direction set arguments");
+    if (phase != RSH_PHASE_THOMSON && phase != RSH_PHASE_HG) return fail(RSH_E_ARG, "unknown phase function");
+    if (phase == RSH_PHASE_HG && !(g > -1.0 && g < 1.0)) return fail(RSH_E_ARG, "HG asymmetry g must be in (-1,1)");
+    if (!normalize && !(full_area > 0)) return fail(RSH_E_ARG, "full_area must be positive");
+    *out = nullptr;
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    auto ds = std::make_unique<rsh_dirset_s>();
+    ds->device = device;
+    ds->m = m;
+    ds->full_area = full_area;
+    ds->phase = phase;
+    ds->g = g;
+    ds->normalize = normalize ? 1 : 0;
+    ds->xyz.assign(xyz, xyz + 3 * (size_t)m);
+    ds->w.assign(w, w + m);
+    auto cleanup = [&](int status) {
+        hipFree(ds->d_dirs);
+        hipFree(ds->d_w);
+        hipFree(ds->d_P);
+        return status;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&ds->d_dirs, 3 * (size_t)m * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&ds->d_w, (size_t)m * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&ds->d_P, (size_t)m * m * sizeof(double))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc(direction set)"));
+    if ((e = hipMemcpy(ds->d_dirs, xyz, 3 * (size_t)m * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ds->d_w, w, (size_t)m * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMemcpy(direction set)"));
+    if ((e = launch_build_phase(ds->d_dirs, ds->d_w, m, full_area, phase, g, ds->normalize, ds->d_P, nullptr)) !=
+            hipSuccess ||
+        (e = hipDeviceSynchronize()) != hipSuccess)
+        return cleanup(hip_fail(e, "build_phase_kernel"));
+    *out = ds.release();
+    return RSH_OK;
+}
+
+int rsh_dirset_destroy(rsh_dirset ds) {
+    if (!ds) return RSH_OK;
+    hipSetDevice(ds->device);
+    hipDeviceSynchronize();
+    for (auto& kv : ds->panels) {
+        hipFree(kv.second.d_panel);
+        hipEventDestroy(kv.second.ready);
+    }
+    hipFree(ds->d_dirs);
+    hipFree(ds->d_w);
+    hipFree(ds->d_P);
+    delete ds;
+    return RSH_OK;
+}
+
+int rsh_dirset_phase_matrix(rsh_dirset ds, int32_t k0, int32_t k1, double* host_out) {
+    if (!ds || !host_out || k0 < 0 || k1 > ds->m || k0 > k1) return fail(RSH_E_ARG, "invalid arguments");
+    RSH_HIP_TRY(hipSetDevice(ds->device));
+    RSH_HIP_TRY(hipMemcpy(host_out, ds->d_P + (int64_t)k0 * ds->m, (size_t)(k1 - k0) * ds->m * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    return RSH_OK;
+}
+
+int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells, int32_t k0,
+                        int32_t k1, double* d_out, int64_t ld_out, void* stream, int32_t kernel,
+                        const rsh_tile* tile) {
+    return scatter_f64(ds, d_illum, faces, n_cells, k0, k1, d_out, ld_out, (hipStream_t)stream, kernel, tile);
+}
+
+int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed, void* stream) {
+    if (count < 0 || (count > 0 && !d_out)) return fail(RSH_E_ARG, "invalid fill arguments");
+    RSH_HIP_TRY(launch_fill_lognormal(d_out, count, mu, sigma, seed, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_selftest_mfma_f64(int32_t device, int32_t* mismatches) {
+    if (!mismatches) return fail(RSH_E_ARG, "null argument");
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    double A[16 * 4], B[4 * 16], D[64 * 4];
+    for (int i = 0; i < 64; i++) {
+        A[i] = 1.0 + i;         // A[r][k] = 1 + 4r + k
+        B[i] = 1.0 / (1 + i);   // B[k][c]
+    }
+    double *dA = nullptr, *dB = nullptr, *dD = nullptr;
+    RSH_HIP_TRY(hipMalloc(&dA, sizeof A));
+    RSH_HIP_TRY(hipMalloc(&dB, sizeof B));
+    RSH_HIP_TRY(hipMalloc(&dD, sizeof D));
+    hipMemcpy(dA, A, sizeof A, hipMemcpyHostToDevice);
+    hipMemcpy(dB, B, sizeof B, hipMemcpyHostToDevice);
+    hipError_t e = launch_mfma_layout_probe(dA, dB, dD, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(D, dD, sizeof D, hipMemcpyDeviceToHost);
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dD);
+    if (e != hipSuccess) return hip_fail(e, "mfma layout probe");
+    int bad = 0;
+    for (int l = 0; l < 64; l++)
+        for (int r = 0; r < 4; r++) {
+            const int row = l / 16 + 4 * r, col = l % 16;  // D layout of v_mfma_f64_16x16x4f64
+            double ref = 0;
+            for (int k = 0; k < 4; k++) ref += A[row * 4 + k] * B[k * 16 + col];
+            if (std::fabs(D[l * 4 + r] - ref) > 1e-12 * std::fabs(ref)) bad++;
+        }
+    *mismatches = bad;
+    return RSH_OK;
+}
+
+// ------------------------------------------------------------------ device memory utilities
+
+int rsh_dev_alloc(int32_t device, int64_t bytes, void** out) {
+    if (!out || bytes < 0) return fail(RSH_E_ARG, "invalid allocation arguments");
+    *out = nullptr;
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    RSH_HIP_TRY(hipMalloc(out, (size_t)std::max<int64_t>(bytes, 1)));
+    return RSH_OK;
+}
+
+int rsh_dev_free(void* ptr) {
+    if (ptr) RSH_HIP_TRY(hipFree(ptr));
+    return RSH_OK;
+}
+
+int rsh_host_alloc_pinned(int64_t bytes, void** out) {
+    if (!out || bytes < 0) return fail(RSH_E_ARG, "invalid allocation arguments");
+    *out = nullptr;
+    RSH_HIP_TRY(hipHostMalloc(out, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault));
+    return RSH_OK;
+}
+
+int rsh_host_free_pinned(void* ptr) {
+    if (ptr) RSH_HIP_TRY(hipHostFree(ptr));
+    return RSH_OK;
+}
+
+static int memcpy_kind(int32_t kind, hipMemcpyKind* k) {
+    if (kind == 0) *k = hipMemcpyHostToDevice;
+    else if (kind == 1) *k = hipMemcpyDeviceToHost;
+    else if (kind == 2) *k = hipMemcpyDeviceToDevice;
+    else return fail(RSH_E_ARG, "memcpy kind must be 0 (H2D), 1 (D2H) or 2 (D2D)");
+    return RSH_OK;
+}
+
+int rsh_dev_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream) {
+    hipMemcpyKind k;
+    if (int rc = memcpy_kind(kind, &k)) return rc;
+    if (bytes < 0) return fail(RSH_E_ARG, "negative byte count");
+    if (bytes == 0) return RSH_OK;
+    if (!dst || !src) return fail(RSH_E_ARG, "null pointer");
+    if (stream) RSH_HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, k, (hipStream_t)stream));
+    else RSH_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, k));
+    return RSH_OK;
+}
+
+int rsh_dev_memcpy2d(void* dst, int64_t dpitch, const void* src, int64_t spitch, int64_t width, int64_t height,
+                     int32_t kind) {
+    hipMemcpyKind k;
+    if (int rc = memcpy_kind(kind, &k)) return rc;
+    if (width < 0 || height < 0 || dpitch < width || spitch < width) return fail(RSH_E_ARG, "invalid 2D copy");
+    if (width == 0 || height == 0) return RSH_OK;
+    if (!dst || !src) return fail(RSH_E_ARG, "null pointer");
+    RSH_HIP_TRY(hipMemcpy2D(dst, (size_t)dpitch, src, (size_t)spitch, (size_t)width, (size_t)height, k));
+    return RSH_OK;
+}
+
+int rsh_dev_fill_f64(double* d, int64_t count, double value, void* stream) {
+    if (count < 0 || (count > 0 && !d)) return fail(RSH_E_ARG, "invalid fill arguments");
+    RSH_HIP_TRY(launch_fill_f64(d, count, value, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_dev_scale_f64(double* d, int64_t count, double factor, void* stream) {
+    if (count < 0 || (count > 0 && !d)) return fail(RSH_E_ARG, "invalid scale arguments");
+    RSH_HIP_TRY(launch_scale_f64(d, count, factor, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_dev_synchronize(int32_t device) {
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    RSH_HIP_TRY(hipDeviceSynchronize());
+    return RSH_OK;
+}
+
+int rsh_dev_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(RSH_E_ARG, "null argument");
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    RSH_HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return RSH_OK;
+}
+
+int rsh_stream_create(int32_t device, void** out) {
+    if (!out) return fail(RSH_E_ARG, "null argument");
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    hipStream_t s = nullptr;
+    RSH_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+    *out = (void*)s;
+    return RSH_OK;
+}
+
+int rsh_stream_destroy(void* stream) {
+    if (stream) RSH_HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_stream_synchronize(void* stream) {
+    RSH_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_event_create(void** out) {
+    if (!out) return fail(RSH_E_ARG, "null argument");
+    hipEvent_t e = nullptr;
+    RSH_HIP_TRY(hipEventCreate(&e));
+    *out = (void*)e;
+    return RSH_OK;
+}
+
+int rsh_event_destroy(void* ev) {
+    if (ev) RSH_HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return RSH_OK;
+}
+
+int rsh_event_record(void* ev, void* stream) {
+    if (!ev) return fail(RSH_E_ARG, "null event");
+    RSH_HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms) {
+    if (!ev_start || !ev_stop || !ms) return fail(RSH_E_ARG, "null argument");
+    RSH_HIP_TRY(hipEventSynchronize((hipEvent_t)ev_stop));
+    RSH_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)ev_start, (hipEvent_t)ev_stop));
+    return RSH_OK;
+}
+
+// ------------------------------------------------------------------ problem API
+
+int rsh_create(const rsh_problem_desc* d, rsh_problem* out) {
+    if (!d || !out) return fail(RSH_E_ARG, "null argument");
+    *out = nullptr;
+    if (d->n_cells < 0 || d->n_dirs <= 0 || (d->faces != 1 && d->faces != 4))
+        return fail(RSH_E_ARG, "invalid problem sizes (faces must be 1 or 4)");
+    if (d->k_begin < 0 || d->k_end > d->n_dirs || d->k_begin > d->k_end)
+        return fail(RSH_E_ARG, "invalid local direction range");
+    rsh_dirset ds = nullptr;
+    int rc = rsh_dirset_create(d->device, d->n_dirs, d->dirs_xyz, d->weights, d->full_area, d->phase, d->g,
+                               d->normalize, &ds);
+    if (rc) return rc;
+    auto p = std::make_unique<rsh_problem_s>();
+    p->device = d->device;
+    p->n = d->n_cells;
+    p->faces = d->faces;
+    p->k_begin = d->k_begin;
+    p->k_end = d->k_end;
+    p->ds = ds;
+    auto cleanup = [&](int status) {
+        hipFree(p->d_illum);
+        hipFree(p->d_S);
+        for (int i = 0; i < RSH_STREAM_COUNT; i++) {
+            if (p->streams[i]) hipStreamDestroy(p->streams[i]);
+            if (p->upload_ev[i]) hipEventDestroy(p->upload_ev[i]);
+            if (p->t0[i]) hipEventDestroy(p->t0[i]);
+            if (p->t1[i]) hipEventDestroy(p->t1[i]);
+        }
+        rsh_dirset_destroy(ds);
+        return status;
+    };
+    const size_t illum_bytes = (size_t)d->faces * d->n_dirs * d->n_cells * sizeof(double);
+    const size_t s_bytes = (size_t)(d->k_end - d->k_begin) * d->n_cells * sizeof(double);
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_illum, std::max<size_t>(illum_bytes, 8))) != hipSuccess ||
+        (e = hipMalloc(&p->d_S, std::max<size_t>(s_bytes, 8))) != hipSuccess)
+        return cleanup(hip_fail(e, "hipMalloc(problem buffers)"));
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) {
+        // Default priority on purpose (see DESIGN notes in the repository).
+        if ((e = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&p->upload_ev[i], hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreate(&p->t0[i])) != hipSuccess || (e = hipEventCreate(&p->t1[i])) != hipSuccess)
+            return cleanup(hip_fail(e, "stream/event creation"));
+    }
+    *out = p.release();
+    return RSH_OK;
+}
+
+int rsh_destroy(rsh_problem p) {
+    if (!p) return RSH_OK;
+    hipSetDevice(p->device);
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) hipStreamSynchronize(p->streams[i]);
+    hipFree(p->d_illum);
+    hipFree(p->d_S);
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) {
+        hipStreamDestroy(p->streams[i]);
+        hipEventDestroy(p->upload_ev[i]);
+        hipEventDestroy(p->t0[i]);
+        hipEventDestroy(p->t1[i]);
+    }
+    rsh_dirset_destroy(p->ds);
+    delete p;
+    return RSH_OK;
+}
+
+static bool valid_stream(int s) { return s >= 0 && s < RSH_STREAM_COUNT; }
+
+int rsh_upload_illum(rsh_problem p, const double* host, int64_t count, int64_t dst_off, int64_t src_off,
+                     int32_t stream) {
+    if (!p || !valid_stream(stream)) return fail(RSH_E_ARG, "invalid problem or stream id");
+    const int64_t total = (int64_t)p->faces * p->ds->m * p->n;
+    if (count < 0 || dst_off < 0 || src_off < 0 || dst_off + count > total)
+        return fail(RSH_E_ARG, "upload range outside device illum");
+    if (count == 0) return RSH_OK;
+    if (!host) return fail(RSH_E_ARG, "null host pointer");
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    RSH_HIP_TRY(hipMemcpyAsync(p->d_illum + dst_off, host + src_off, (size_t)count * sizeof(double),
+                               hipMemcpyHostToDevice, p->streams[stream]));
+    RSH_HIP_TRY(hipEventRecord(p->upload_ev[stream], p->streams[stream]));
+    p->upload_pending[stream] = true;
+    return RSH_OK;
+}
+
+int rsh_copy_illum_sync(rsh_problem p, const double* host, int64_t count) {
+    if (!p) return fail(RSH_E_ARG, "null problem");
+    const int64_t total = (int64_t)p->faces * p->ds->m * p->n;
+    if (count < 0 || count > total) return fail(RSH_E_ARG, "copy larger than device illum");
+    if (count == 0) return RSH_OK;
+    if (!host) return fail(RSH_E_ARG, "null host pointer");
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) RSH_HIP_TRY(hipStreamSynchronize(p->streams[i]));
+    RSH_HIP_TRY(hipMemcpy(p->d_illum, host, (size_t)count * sizeof(double), hipMemcpyHostToDevice));
+    return RSH_OK;
+}
+
+int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* host_out) {
+    if (!p || !valid_stream(stream)) return fail(RSH_E_ARG, "invalid problem or stream id");
+    if (k0 < p->k_begin || k1 > p->k_end || k0 > k1)
+        return fail(RSH_E_ARG, "direction range outside the problem's local range");
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    hipStream_t st = p->streams[stream];
+    // Order every earlier upload (on any stream) before this launch.
+    for (int i = 0; i < RSH_STREAM_COUNT; i++)
+        if (p->upload_pending[i] && i != stream) RSH_HIP_TRY(hipStreamWaitEvent(st, p->upload_ev[i], 0));
+    double* dst = p->d_S + (int64_t)(k0 - p->k_begin) * p->n;
+    RSH_HIP_TRY(hipEventRecord(p->t0[stream], st));
+    int rc = scatter_f64(p->ds, p->d_illum, p->faces, p->n, k0, k1, dst, p->n, st, RSH_F64_AUTO, nullptr);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipEventRecord(p->t1[stream], st));
+    p->timed[stream] = true;
+    if (host_out && k1 > k0 && p->n > 0)
+        RSH_HIP_TRY(hipMemcpyAsync(host_out + (int64_t)(k0 - p->k_begin) * p->n, dst,
+                                   (size_t)(k1 - k0) * p->n * sizeof(double), hipMemcpyDeviceToHost, st));
+    return RSH_OK;
+}
+
+int rsh_sync(rsh_problem p, int32_t stream) {
+    if (!p) return fail(RSH_E_ARG, "null problem");
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    if (stream < 0) {
+        RSH_HIP_TRY(hipDeviceSynchronize());
+        return RSH_OK;
+    }
+    if (!valid_stream(stream)) return fail(RSH_E_ARG, "invalid stream id");
+    RSH_HIP_TRY(hipStreamSynchronize(p->streams[stream]));
+    return RSH_OK;
+}
+
+int rsh_problem_buffers(rsh_problem p, double** d_illum, double** d_scattering) {
+    if (!p) return fail(RSH_E_ARG, "null problem");
+    if (d_illum) *d_illum = p->d_illum;
+    if (d_scattering) *d_scattering = p->d_S;
+    return RSH_OK;
+}
+
+int rsh_problem_stream(rsh_problem p, int32_t stream, void** hip_stream) {
+    if (!p || !hip_stream || !valid_stream(stream)) return fail(RSH_E_ARG, "invalid arguments");
+    *hip_stream = (void*)p->streams[stream];
+    return RSH_OK;
+}
+
+int rsh_event_times(rsh_problem p, float* ms) {
+    if (!p || !ms) return fail(RSH_E_ARG, "null argument");
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) {
+        ms[i] = -1.0f;
+        if (!p->timed[i]) continue;
+        RSH_HIP_TRY(hipEventSynchronize(p->t1[i]));
+        RSH_HIP_TRY(hipEventElapsedTime(&ms[i], p->t0[i], p->t1[i]));
+    }
+    return RSH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// FP64 kernels of the scattering integral on CDNA4 (gfx950):
+//   * phase-matrix build  P[k][j] = p(w_k . w_j) w_j / den_k   (once per direction set)
+//   * panel re-layout     P rows [k0,k1) -> MFMA A-fragment order (cached per range)
+//   * direct kernel       one thread per (cell, output direction), reference order
+//   * MFMA GEMM           S[k][c] = sum_j P[k][j] * sum_f I[j][c][f]
+//
+// The contraction is S (R x N) = P (R x M) * X (M x N) with X[j][c] the face sum
+// of illum[j][c][0..F). On MI355X it is compute-bound in FP64 (2M flops per
+// (cell, direction) pair against 32 B of face input), so it runs on
+// v_mfma_f64_16x16x4f64 with:
+//   - one wave per 16 consecutive cells, holding ALL output rows of a pass in
+//     accumulators (25 row blocks = 400 directions -> 100 doubles per lane), so
+//     every face value is read from HBM exactly once per pass;
+//   - the face sum fused into the B-operand load: lane (jr, cl) reads the F
+//     contiguous faces of cell cl for direction 4s + jr (a 16-lane group reads
+//     16*F*8 contiguous bytes), sums them in FP64 and feeds the sum to MFMA;
+//   - the phase panel streamed through LDS in double-buffered chunks shared by
+//     all waves of the workgroup (A operand, one conflict-free ds_read_b64 per
+//     fragment), prefetched from L2 one chunk ahead;
+//   - the epilogue storing each 16x16 tile as four 128-byte row segments of the
+//     direction-major output with non-temporal stores (S is not re-read).
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// No FMA contraction: keeps the reference's rounding (3 (1 + mu^2)) / 4 exactly.
+__host__ __device__ inline double phase_kernel(int kind, double g, double mu) {
+#pragma clang fp contract(off)
+    if (kind == RSH_PHASE_THOMSON) return (3. * (1 + mu * mu)) / 4;  // cuda/cuda_utils.cu:15
+    const double d = 1.0 + g * g - 2.0 * g * mu;
+    return (1.0 - g * g) / (d * sqrt(d));
+}
+
+// ---------------------------------------------------------------------------
+// Phase matrix. One 256-thread block per output row k; the FP64 row sum for
+// normalisation is a block tree reduction.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void build_phase_kernel(const double* __restrict__ dirs,
+                                                          const double* __restrict__ w, int m,
+                                                          double full_area, int kind, double g,
+                                                          int normalize, double* __restrict__ P) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x;
+    const double xk = dirs[3 * k], yk = dirs[3 * k + 1], zk = dirs[3 * k + 2];
+    double* row = P + (int64_t)k * m;
+    double part = 0;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        // mu in the order of the reference's Gamma loop (cuda_utils.cu:9-13)
+        const double mu = (dirs[3 * j] * xk + dirs[3 * j + 1] * yk) + dirs[3 * j + 2] * zk;
+        const double v = phase_kernel(kind, g, mu) * w[j];
+        row[j] = v;
+        part += v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double den = normalize ? red[0] : full_area;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) row[j] = row[j] / den;
+}
+
+hipError_t launch_build_phase(const double* d_dirs, const double* d_w, int m, double full_area, int kind,
+                              double g, int normalize, double* d_P, hipStream_t st) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(build_phase_kernel, dim3(m), dim3(256), 0, st, d_dirs, d_w, m, full_area, kind, g,
+                       normalize, d_P);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Panel re-layout (see rsh_internal.hpp for the fragment-major layout).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void panel_f64_kernel(const double* __restrict__ P, int m, int k0, int k1,
+                                                        double scale, int rb_per_pass, int steps, int64_t total,
+                                                        double* __restrict__ panel) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx & 63);
+    int64_t frag = idx >> 6;
+    const int rb_local = (int)(frag % rb_per_pass);
+    frag /= rb_per_pass;
+    const int s = (int)(frag % steps);
+    const int pass = (int)(frag / steps);
+    const int k = k0 + (pass * rb_per_pass + rb_local) * 16 + (lane & 15);
+    const int j = 4 * s + (lane >> 4);
+    double v = 0.0;
+    if (k < k1 && j < m) v = P[(int64_t)k * m + j] * scale;
+    panel[idx] = v;
+}
+
+static const int kRbSet[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 13, 16, 20, 25};
+
+F64Plan plan_f64(int m, int rows) {
+    F64Plan pl{};
+    const int nrb = (rows + 15) / 16;
+    pl.passes = nrb > 0 ? (nrb + kMaxRowBlocksPerPass - 1) / kMaxRowBlocksPerPass : 0;
+    const int per = pl.passes > 0 ? (nrb + pl.passes - 1) / pl.passes : 0;
+    pl.rb_per_pass = 0;
+    for (int v : kRbSet)
+        if (v >= per) { pl.rb_per_pass = v; break; }
+    pl.rb_total = pl.rb_per_pass * pl.passes;
+    const int steps = (m + 3) / 4;
+    pl.steps = (steps + kPanelStepAlign - 1) / kPanelStepAlign * kPanelStepAlign;
+    return pl;
+}
+
+int64_t panel_f64_doubles(const F64Plan& pl) { return (int64_t)pl.steps * pl.rb_total * 64; }
+
+hipError_t launch_panel_f64(const double* d_P, int m, int k0, int k1, double scale, const F64Plan& pl,
+                            double* d_panel, hipStream_t st) {
+    const int64_t total = panel_f64_doubles(pl);
+    if (total == 0) return hipSuccess;
+    const int64_t blocks = (total + 255) / 256;
+    hipLaunchKernelGGL(panel_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d_P, m, k0, k1, scale,
+                       pl.rb_per_pass, pl.steps, total, d_panel);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Direct kernel: thread = (cell, output direction). With ref_order set it
+// evaluates Gamma inline and accumulates ((Gamma * Ibar) * w_j) for j
+// ascending, then divides by F, exactly as GetS (Illum_async.cpp:212-231) and
+// d_GetS_MPI_Stream (cuda_kernel.cu:98-110) with no FMA contraction, so its
+// result is bit-identical to a non-contracted sequential CPU evaluation.
+// Otherwise it accumulates P[k][j] * Ibar in the same order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void direct_f64_kernel(const double* __restrict__ illum, int faces, int64_t n,
+                                                         int m, const double* __restrict__ dirs,
+                                                         const double* __restrict__ w, double full_area,
+                                                         const double* __restrict__ P, int ref_order, int k0,
+                                                         double* __restrict__ out, int64_t ld) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = k0 + (int)blockIdx.y;
+    if (c >= n) return;
+    const double xk = dirs[3 * k], yk = dirs[3 * k + 1], zk = dirs[3 * k + 2];
+    const int64_t stride = n * faces;
+    const double* src = illum + c * faces;
+    double s = 0;
+    for (int j = 0; j < m; j++) {
+        double I = 0;
+        for (int f = 0; f < faces; f++) I += src[f];
+        I /= faces;
+        if (ref_order) {
+            const double mu = (dirs[3 * j] * xk + dirs[3 * j + 1] * yk) + dirs[3 * j + 2] * zk;
+            s += phase_kernel(RSH_PHASE_THOMSON, 0.0, mu) * I * w[j];
+        } else {
+            s += P[(int64_t)k * m + j] * I;
+        }
+        src += stride;
+    }
+    out[(int64_t)blockIdx.y * ld + c] = ref_order ? s / full_area : s;
+}
+
+hipError_t launch_direct_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_dirs,
+                             const double* d_w, double full_area, const double* d_P, int ref_order, int k0,
+                             int k1, double* d_out, int64_t ld, hipStream_t st) {
+    if (n <= 0 || k1 <= k0) return hipSuccess;
+    const int64_t bx = (n + 255) / 256;
+    hipLaunchKernelGGL(direct_f64_kernel, dim3((unsigned)bx, (unsigned)(k1 - k0)), dim3(256), 0, st, d_illum,
+                       faces, n, m, d_dirs, d_w, full_area, d_P, ref_order, k0, d_out, ld);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA GEMM.
+//
+// Operand layouts of v_mfma_f64_16x16x4f64 (D = A * B + C, 16x16x4):
+//   A: lane l holds A[l % 16][l / 16]      (row = output direction, col = j)
+//   B: lane l holds B[l / 16][l % 16]      (row = j, col = cell)
+//   D: lane l, item r holds D[l / 16 + 4 r][l % 16]   (rows interleaved by 4)
+// (measured with one-hot operands on gfx950; rsh_selftest_mfma_f64 re-checks it.)
+//
+// Workgroup = W waves = 16 W consecutive cells; grid.y = pass over row blocks.
+// Loop over chunks of SC j-steps: the panel chunk (SC * RB fragments) is staged
+// through registers into one of two LDS buffers while the other is consumed.
+// ---------------------------------------------------------------------------
+template <int RB, int W, int SC, int FACES>
+__global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restrict__ illum, int64_t n, int m,
+                                                          const double* __restrict__ panel, int steps,
+                                                          double* __restrict__ out, int64_t ld, int rows) {
+    constexpr int FRAG = 64;                       // doubles per fragment
+    constexpr int UNITS = SC * RB * FRAG / 2;      // 16-byte units per LDS stage
+    constexpr int THREADS = 64 * W;
+    constexpr int PER = (UNITS + THREADS - 1) / THREADS;
+    constexpr bool TAIL = (UNITS % THREADS) != 0;
+    __shared__ v2d lds[2][UNITS];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int jr = lane >> 4;
+    const int cl = lane & 15;
+    const int64_t cell = (int64_t)blockIdx.x * (16 * W) + wave * 16 + cl;
+    const bool cvalid = cell < n;
+    const int rb0 = blockIdx.y * RB;
+
+    // Panel of this pass: steps * RB fragments, stage t = units [t*UNITS, (t+1)*UNITS).
+    const v2d* psrc = reinterpret_cast<const v2d*>(panel + (int64_t)blockIdx.y * steps * RB * FRAG) + tid;
+    // Face pointer of (j = jr, cell); one j-step advances 4 directions.
+    const double* xp = illum + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
+    const int64_t xstep = 4 * n * FACES;
+
+    v4d acc[RB];
+#pragma unroll
+    for (int i = 0; i < RB; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    v2d preg[PER];
+    double xr[SC][FACES];
+    double xb[SC];
+
+#define RSH_LOAD_PANEL(t)                                                     \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
+            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];               \
+    }
+#define RSH_STORE_PANEL(buf)                                                  \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
+            lds[buf][tid + i * THREADS] = preg[i];                            \
+    }
+#define RSH_LOAD_X(t)                                                         \
+    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
+        const int step = (t) * SC + s;                                        \
+        if (cvalid && 4 * step + jr < m) {                                    \
+            const double* p = xp + (int64_t)step * xstep;                     \
+            if constexpr (FACES == 4) {                                       \
+                const v2d a = reinterpret_cast<const v2d*>(p)[0];     \
+                const v2d b = reinterpret_cast<const v2d*>(p)[1];     \
+                xr[s][0] = a.x; xr[s][1 % FACES] = a.y;                       \
+                xr[s][2 % FACES] = b.x; xr[s][3 % FACES] = b.y;               \
+            } else {                                                          \
+                xr[s][0] = p[0];                                              \
+            }                                                                 \
+        } else {                                                              \
+            _Pragma("unroll") for (int f = 0; f < FACES; f++) xr[s][f] = 0.0; \
+        }                                                                     \
+    }
+#define RSH_FACE_SUM()                                                        \
+    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
+        double v = xr[s][0];                                                  \
+        _Pragma("unroll") for (int f = 1; f < FACES; f++) v += xr[s][f];      \
+        xb[s] = v;                                                            \
+    }
+
+    const int nchunks = steps / SC;
+    RSH_LOAD_PANEL(0)
+    RSH_LOAD_X(0)
+    RSH_STORE_PANEL(0)
+    RSH_FACE_SUM()
+    __syncthreads();
+
+    for (int t = 0; t < nchunks; t++) {
+        const int buf = t & 1;
+        const bool more = t + 1 < nchunks;
+        if (more) {
+            RSH_LOAD_PANEL(t + 1)
+            RSH_LOAD_X(t + 1)
+        }
+        const double* a_base = reinterpret_cast<const double*>(lds[buf]) + lane;
+#pragma unroll
+        for (int s = 0; s < SC; s++) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const double a = a_base[(s * RB + rb) * FRAG];
+                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[s], acc[rb], 0, 0, 0);
+            }
+        }
+        if (more) {
+            RSH_STORE_PANEL(buf ^ 1)
+            RSH_FACE_SUM()
+        }
+        __syncthreads();
+    }
+#undef RSH_LOAD_PANEL
+#undef RSH_STORE_PANEL
+#undef RSH_LOAD_X
+#undef RSH_FACE_SUM
+
+    if (!cvalid) return;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = (rb0 + rb) * 16 + jr + 4 * r;
+            if (row < rows) __builtin_nontemporal_store(acc[rb][r], out + (int64_t)row * ld + cell);
+        }
+    }
+}
+
+template <int W, int SC, int FACES>
+static hipError_t launch_mfma_rb(int rb, dim3 grid, hipStream_t st, const double* illum, int64_t n, int m,
+                                 const double* panel, int steps, double* out, int64_t ld, int rows) {
+#define RSH_RB_CASE(V)                                                                                      \
+    case V:                                                                                                 \
+        hipLaunchKernelGGL((mfma_f64_kernel<V, W, SC, FACES>), grid, dim3(64 * W), 0, st, illum, n, m, panel, \
+                           steps, out, ld, rows);                                                           \
+        break;
+    switch (rb) {
+        RSH_RB_CASE(1) RSH_RB_CASE(2) RSH_RB_CASE(3) RSH_RB_CASE(4) RSH_RB_CASE(5) RSH_RB_CASE(6)
+        RSH_RB_CASE(7) RSH_RB_CASE(8) RSH_RB_CASE(10) RSH_RB_CASE(13) RSH_RB_CASE(16) RSH_RB_CASE(20)
+        RSH_RB_CASE(25)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef RSH_RB_CASE
+    return hipGetLastError();
+}
+
+bool mfma_f64_tile_supported(int waves, int chunk_steps) {
+    // (8 waves, 4 steps) is left out: at 25 row blocks it exceeds 256 VGPRs and spills.
+    return (waves == 4 && (chunk_steps == 2 || chunk_steps == 4)) || (waves == 8 && chunk_steps == 2);
+}
+
+hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_panel,
+                           const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves, int chunk_steps,
+                           hipStream_t st) {
+    if (n <= 0 || rows <= 0) return hipSuccess;
+    if (!mfma_f64_tile_supported(waves, chunk_steps) || (faces != 1 && faces != 4)) return hipErrorInvalidValue;
+    if (pl.steps % chunk_steps != 0) return hipErrorInvalidValue;
+    const int64_t bx = (n + 16 * waves - 1) / (16 * waves);
+    const dim3 grid((unsigned)bx, (unsigned)pl.passes);
+    const int rb = pl.rb_per_pass;
+#define RSH_TILE(W, SC)                                                                                \
+    if (waves == W && chunk_steps == SC) {                                                             \
+        if (faces == 4)                                                                                \
+            return launch_mfma_rb<W, SC, 4>(rb, grid, st, d_illum, n, m, d_panel, pl.steps, d_out, ld, rows); \
+        return launch_mfma_rb<W, SC, 1>(rb, grid, st, d_illum, n, m, d_panel, pl.steps, d_out, ld, rows);    \
+    }
+    RSH_TILE(8, 2)
+    RSH_TILE(4, 2)
+    RSH_TILE(4, 4)
+#undef RSH_TILE
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Layout probe: one MFMA on a single wave with A, B given in natural order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void mfma_layout_probe_kernel(const double* __restrict__ A,  // [16][4]
+                                                               const double* __restrict__ B,  // [4][16]
+                                                               double* __restrict__ D) {      // [64][4]
+    const int l = threadIdx.x;
+    const double a = A[(l % 16) * 4 + l / 16];
+    const double b = B[(l / 16) * 16 + l % 16];
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; r++) D[l * 4 + r] = acc[r];
+}
+
+hipError_t launch_mfma_layout_probe(const double* d_a, const double* d_b, double* d_d, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, st, d_a, d_b, d_d);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Seeded log-normal fill (SplitMix64 hash of (seed, index), Box-Muller).
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void fill_lognormal_kernel(double* __restrict__ out, int64_t count, double mu,
+                                                             double sigma, uint64_t seed) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t h1 = splitmix64(seed ^ splitmix64(2 * (uint64_t)i));
+        const uint64_t h2 = splitmix64(seed ^ splitmix64(2 * (uint64_t)i + 1));
+        const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+        const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        out[i] = exp(mu + sigma * z);
+    }
+}
+
+hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
+                                 hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_lognormal_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d, count, mu, sigma,
+                       seed);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fill / scale utilities (kernels, so that they are safe inside captured graphs).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_f64_kernel(double* __restrict__ d, int64_t count, double value) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) d[i] = value;
+}
+
+__global__ __launch_bounds__(256) void scale_f64_kernel(double* __restrict__ d, int64_t count, double factor) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) d[i] *= factor;
+}
+
+static unsigned grid_for(int64_t count) {
+    int64_t b = (count + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_fill_f64(double* d, int64_t count, double value, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_for(count)), dim3(256), 0, st, d, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_f64(double* d, int64_t count, double factor, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_f64_kernel, dim3(grid_for(count)), dim3(256), 0, st, d, count, factor);
+    return hipGetLastError();
+}
+
+}  // namespace rsh

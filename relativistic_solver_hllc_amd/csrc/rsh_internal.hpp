@@ -1,0 +1,68 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/rsh_scatter.h"
+
+namespace rsh {
+
+// Thread-local error message behind rsh_last_error().
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define RSH_HIP_TRY(expr)                                     \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return ::rsh::hip_fail(_e, #expr); \
+    } while (0)
+
+// ------------------------------------------------------------------ layout
+// FP64 phase panel (A operand of v_mfma_f64_16x16x4f64), fragment-major:
+//   panel[((pass * steps + s) * rb_per_pass + rb) * 64 + lane]
+//       = P[k0 + 16 (pass * rb_per_pass + rb) + lane % 16][4 s + lane / 16] * scale
+// for j-steps s (4 input directions each) and row blocks rb (16 output
+// directions each); zero outside [k0,k1) x [0,M). One (s, rb) fragment is 512
+// contiguous bytes, so a wave reads it with one coalesced load / LDS read.
+constexpr int kPanelStepAlign = 4;      // j-steps padded to a multiple of this
+constexpr int kMaxRowBlocksPerPass = 25; // 400 output directions per pass
+
+struct F64Plan {
+    int rb_per_pass;   // row blocks per workgroup pass (template value)
+    int passes;        // grid.y
+    int rb_total;      // rb_per_pass * passes (panel rows / 16)
+    int steps;         // padded j-steps (multiple of kPanelStepAlign)
+};
+
+F64Plan plan_f64(int m, int rows);
+int64_t panel_f64_doubles(const F64Plan& pl);
+
+// ------------------------------------------------------------------ kernels
+hipError_t launch_build_phase(const double* d_dirs, const double* d_w, int m, double full_area,
+                              int kind, double g, int normalize, double* d_P, hipStream_t st);
+
+hipError_t launch_panel_f64(const double* d_P, int m, int k0, int k1, double scale, const F64Plan& pl,
+                            double* d_panel, hipStream_t st);
+
+hipError_t launch_direct_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_dirs,
+                             const double* d_w, double full_area, const double* d_P, int ref_order,
+                             int k0, int k1, double* d_out, int64_t ld, hipStream_t st);
+
+bool mfma_f64_tile_supported(int waves, int chunk_steps);
+hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_panel,
+                           const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves,
+                           int chunk_steps, hipStream_t st);
+
+hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
+                                 hipStream_t st);
+
+hipError_t launch_fill_f64(double* d, int64_t count, double value, hipStream_t st);
+hipError_t launch_scale_f64(double* d, int64_t count, double factor, hipStream_t st);
+
+hipError_t launch_mfma_layout_probe(const double* d_a, const double* d_b, double* d_d, hipStream_t st);
+
+}  // namespace rsh

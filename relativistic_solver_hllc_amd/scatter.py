@@ -1,0 +1,209 @@
+"""Python host mirror of the scattering-integral boundary.
+
+Thin wrappers over the C ABI (include/rsh_scatter.h). Naming follows the
+reference's host API (solver_hllc_illum_cuda/cuda/cuda_solve.h:5-37):
+
+    reference                                   here
+    ReadSphereDirectionDecartToSpherical        read_directions
+    GetSend / GetDisp                           partition
+    grid_directions_init_copy_host_to_device    DirectionSet
+    InitDevice / ClearDevice                    Problem(...) / Problem.close()
+    CudaSendIllumAsync / CopyIllumOnDevice      Problem.send_illum_async / copy_illum_on_device
+    CalculateIntScattering*                     Problem.calculate_int_scattering
+    CudaSyncStream / CudaWait                   Problem.sync_stream / wait
+    CudaEventPrint                              Problem.event_times
+
+Device memory handed to the *_dev functions is any object with a device
+address (torch CUDA tensors are used by the tests and the benchmark).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _native as N
+
+PHASES = {"thomson": 0, "hg": 1}
+F64_KERNELS = {"auto": 0, "direct": 1, "mfma": 2}
+STREAM_SCATTERING_1, STREAM_SCATTERING_2, STREAM_PARAMS = 0, 1, 2
+
+
+def _ptr(a) -> int:
+    """Device or host address of a torch tensor or numpy array."""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a)
+
+
+def _f64c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+# ------------------------------------------------------------------ host helpers
+
+def fibonacci_directions(m: int) -> Tuple[np.ndarray, np.ndarray, float]:
+    xyz = np.empty((m, 3))
+    w = np.empty(m)
+    F = C.c_double()
+    N.call("rsh_fibonacci_directions", m, xyz.ctypes.data, w.ctypes.data, C.byref(F))
+    return xyz, w, F.value
+
+
+def read_directions(path: str, cap: int = 1 << 16) -> Tuple[np.ndarray, np.ndarray, float]:
+    xyz = np.empty((cap, 3))
+    w = np.empty(cap)
+    m = C.c_int32()
+    F = C.c_double()
+    N.call("rsh_read_directions", path.encode(), cap, C.byref(m), xyz.ctypes.data, w.ctypes.data, C.byref(F))
+    return xyz[: m.value].copy(), w[: m.value].copy(), F.value
+
+
+def write_directions(path: str, xyz, w, full_area: float) -> None:
+    xyz, w = _f64c(xyz), _f64c(w)
+    N.call("rsh_write_directions", path.encode(), len(w), xyz.ctypes.data, w.ctypes.data, float(full_area))
+
+
+def partition(np_: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
+    send = np.empty(np_, dtype=np.int32)
+    disp = np.empty(np_, dtype=np.int32)
+    N.call("rsh_partition", np_, n, send.ctypes.data, disp.ctypes.data)
+    return send, disp
+
+
+def device_count() -> int:
+    n = C.c_int32()
+    N.call("rsh_device_count", C.byref(n))
+    return n.value
+
+
+def selftest_mfma_f64(device: int = 0) -> int:
+    bad = C.c_int32()
+    N.call("rsh_selftest_mfma_f64", device, C.byref(bad))
+    return bad.value
+
+
+def fill_lognormal_dev(out, mu: float, sigma: float, seed: int, stream=None) -> None:
+    N.call("rsh_fill_lognormal_dev", _ptr(out), out.numel(), float(mu), float(sigma), int(seed) & (2**64 - 1),
+           _stream(stream))
+
+
+# ------------------------------------------------------------------ direction set
+
+class DirectionSet:
+    """Directions, weights and the device FP64 phase matrix of one direction set."""
+
+    def __init__(self, xyz, w, full_area: float, phase: str = "thomson", g: float = 0.0,
+                 normalize: bool = False, device: int = 0):
+        self.xyz, self.w = _f64c(xyz).reshape(-1, 3), _f64c(w)
+        self.m = len(self.w)
+        self.full_area = float(full_area)
+        self.phase, self.g, self.normalize, self.device = phase, float(g), bool(normalize), device
+        h = C.c_void_p()
+        N.call("rsh_dirset_create", device, self.m, self.xyz.ctypes.data, self.w.ctypes.data, self.full_area,
+               PHASES[phase], self.g, int(self.normalize), C.byref(h))
+        self.handle = h
+
+    def phase_matrix(self, k0: int = 0, k1: Optional[int] = None) -> np.ndarray:
+        k1 = self.m if k1 is None else k1
+        out = np.empty((k1 - k0, self.m))
+        N.call("rsh_dirset_phase_matrix", self.handle, k0, k1, out.ctypes.data)
+        return out
+
+    def scatter_f64(self, illum, faces: int, n_cells: int, k0: int, k1: int, out, ld: Optional[int] = None,
+                    stream=None, kernel: str = "auto", tile: Optional[Tuple[int, int]] = None) -> None:
+        """S[k - k0][c] = sum_j P[k][j] * mean_f illum[j][c][f] on device memory."""
+        t = C.byref(N.Tile(*tile)) if tile is not None else None
+        N.call("rsh_scatter_f64_dev", self.handle, _ptr(illum), faces, n_cells, k0, k1, _ptr(out),
+               n_cells if ld is None else ld, _stream(stream), F64_KERNELS[kernel], t)
+
+    def close(self) -> None:
+        if self.handle:
+            N.call("rsh_dirset_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ problem (InitDevice ...)
+
+class Problem:
+    """Reference-shaped problem: device illum/scattering buffers + 3 streams."""
+
+    def __init__(self, n_cells: int, xyz, w, full_area: float, k_begin: int = 0, k_end: Optional[int] = None,
+                 faces: int = 4, phase: str = "thomson", g: float = 0.0, normalize: bool = False, device: int = 0):
+        xyz, w = _f64c(xyz).reshape(-1, 3), _f64c(w)
+        m = len(w)
+        k_end = m if k_end is None else k_end
+        self.n_cells, self.m, self.faces, self.k_begin, self.k_end = n_cells, m, faces, k_begin, k_end
+        d = N.ProblemDesc(device, n_cells, faces, m, xyz.ctypes.data, w.ctypes.data, float(full_area), k_begin,
+                          k_end, PHASES[phase], float(g), int(normalize))
+        h = C.c_void_p()
+        N.call("rsh_create", C.byref(d), C.byref(h))
+        self.handle = h
+
+    def send_illum_async(self, host, count: int, dst_off: int = 0, src_off: Optional[int] = None,
+                         stream: int = STREAM_SCATTERING_1) -> None:
+        """CudaSendIllumAsync(size, shift[, shift_host], Illum_host)."""
+        src_off = dst_off if src_off is None else src_off
+        N.call("rsh_upload_illum", self.handle, _ptr(host), count, dst_off, src_off, stream)
+
+    def copy_illum_on_device(self, host, count: int) -> None:
+        N.call("rsh_copy_illum_sync", self.handle, _ptr(host), count)
+
+    def calculate_int_scattering(self, k0: int, k1: int, stream: int = STREAM_SCATTERING_1, host_out=None) -> None:
+        N.call("rsh_scatter", self.handle, k0, k1, stream, None if host_out is None else _ptr(host_out))
+
+    def sync_stream(self, stream: int) -> None:
+        N.call("rsh_sync", self.handle, stream)
+
+    def wait(self) -> None:
+        N.call("rsh_sync", self.handle, -1)
+
+    def buffers(self) -> Tuple[int, int]:
+        a, b = C.c_void_p(), C.c_void_p()
+        N.call("rsh_problem_buffers", self.handle, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def event_times(self) -> np.ndarray:
+        ms = (C.c_float * 3)()
+        N.call("rsh_event_times", self.handle, ms)
+        return np.array(list(ms))
+
+    def close(self) -> None:
+        if self.handle:
+            N.call("rsh_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

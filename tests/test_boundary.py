@@ -1,0 +1,118 @@
+"""CPU tests of the C-ABI boundary (no GPU compute calls).
+
+* librsh_scatter.so loads and exports every function include/*.h declares;
+* host-only entry points (direction generator, direction file I/O, partition)
+  agree with the oracle's restatement of the reference;
+* invalid arguments come back as status codes with a message, before any
+  device work is attempted.
+"""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from relativistic_solver_hllc_amd import _native as N
+from relativistic_solver_hllc_amd import scatter as rs
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"\b(rsh_[A-Za-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "rsh_scatter" in names and "rsh_create" in names and len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    missing = [n for n in declared_functions() if not hasattr(N.lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    # and the Python mirror binds every one of them
+    assert set(declared_functions()) <= set(N.PROTOTYPES)
+
+
+def test_abi_version():
+    assert N.lib.rsh_abi_version() == 1
+
+
+@pytest.mark.parametrize("m", [1, 2, 26, 200, 400, 800])
+def test_fibonacci_matches_oracle(m):
+    xyz, w, F = rs.fibonacci_directions(m)
+    oxyz, ow, oF = orc.fibonacci(m)
+    assert np.array_equal(xyz, oxyz) and np.array_equal(w, ow) and F == oF
+
+
+def test_direction_file_roundtrip(tmp_path, golden_dir):
+    xyz, w, F = rs.read_directions(os.path.join(golden_dir, "dirs_M26.txt"))
+    oxyz, ow, oF = orc.read_directions(os.path.join(golden_dir, "dirs_M26.txt"))
+    assert np.array_equal(xyz, oxyz) and np.array_equal(w, ow) and F == oF
+    p = str(tmp_path / "d.txt")
+    rs.write_directions(p, xyz, w, F)
+    x2, w2, F2 = rs.read_directions(p)
+    assert np.array_equal(x2, xyz) and np.array_equal(w2, w) and F2 == F
+    # the oracle reads what the library writes
+    x3, w3, F3 = orc.read_directions(p)
+    assert np.array_equal(x3, xyz) and F3 == F
+
+
+def test_direction_file_errors(tmp_path):
+    with pytest.raises(N.RshError) as e:
+        rs.read_directions(str(tmp_path / "missing.txt"))
+    assert e.value.status == 5
+    bad = tmp_path / "bad.txt"
+    bad.write_text("3\n1 0 0 1\n1 0 1\n")
+    with pytest.raises(N.RshError) as e:
+        rs.read_directions(str(bad))
+    assert e.value.status == 5
+    big = tmp_path / "big.txt"
+    big.write_text("5\n" + "1 0 0 1\n" * 5 + "5\n")
+    with pytest.raises(N.RshError) as e:
+        rs.read_directions(str(big), cap=4)
+    assert e.value.status == 1
+
+
+def test_partition_matches_oracle(golden_dir):
+    for p in range(1, 17):
+        for n in (0, 1, 5, 126, 364, 400, 401, 800):
+            send, disp = rs.partition(p, n)
+            assert np.array_equal(send, orc.get_send(p, n))
+            assert np.array_equal(disp, orc.get_disp(p, n))
+    with pytest.raises(N.RshError):
+        rs.partition(0, 5)
+
+
+def test_argument_errors_without_device():
+    null = C.c_void_p()
+    # null direction set
+    assert N.lib.rsh_scatter_f64_dev(null, null, 4, 10, 0, 1, null, 10, null, 0, None) == 1
+    assert b"direction set" in N.lib.rsh_last_error()
+    # bad problem descriptor (faces = 3) is rejected before any device call
+    d = N.ProblemDesc(0, 10, 3, 4, None, None, 1.0, 0, 4, 0, 0.0, 0)
+    h = C.c_void_p()
+    assert N.lib.rsh_create(C.byref(d), C.byref(h)) == 1
+    d = N.ProblemDesc(0, 10, 4, 4, None, None, 1.0, 2, 1, 0, 0.0, 0)
+    assert N.lib.rsh_create(C.byref(d), C.byref(h)) == 1
+    assert N.lib.rsh_scatter(null, 0, 1, 0, None) == 1
+    assert N.lib.rsh_sync(null, 0) == 1
+    assert N.lib.rsh_upload_illum(null, None, 1, 0, 0, 0) == 1
+    xyz, w, F = rs.fibonacci_directions(4)
+    # HG with |g| >= 1 and unknown phase ids are argument errors
+    assert N.lib.rsh_dirset_create(0, 4, xyz.ctypes.data, w.ctypes.data, F, 1, 1.0, 1, C.byref(h)) == 1
+    assert N.lib.rsh_dirset_create(0, 4, xyz.ctypes.data, w.ctypes.data, F, 7, 0.0, 0, C.byref(h)) == 1
+
+
+def test_device_count_is_callable():
+    assert rs.device_count() >= 0
