@@ -66,6 +66,62 @@ def cpu_baseline(illum_dev, xyz, w, F, n, m, threads, k_slice, variant):
     }
 
 
+def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cells):
+    """Config 3: same mesh and intensities, HG g = 0.8 with FP64 row normalisation;
+    FP64 MFMA vs the split forms (state packed once, untimed; FP32 MFMA accumulation)."""
+    from oracle import orc
+
+    res = {"workload": "config 3: same 1,572,864-cell x 4-face mesh and intensities, M=400, "
+                       "Henyey-Greenstein g=0.8, FP64 row normalisation",
+           "phase": "hg", "g": 0.8, "forms": {}}
+    ds = rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True)
+    e0, e1 = D.Event(), D.Event()
+    for _ in range(warmup):
+        ds.scatter_f64(illum, 4, n, 0, m, out, stream=stream)
+    e0.record(stream)
+    for _ in range(steps):
+        ds.scatter_f64(illum, 4, n, 0, m, out, stream=stream)
+    e1.record(stream)
+    t64 = e0.elapsed_ms(e1) / steps
+    res["fp64_ms"] = t64
+    wide = np.arange(0, n, 997)
+    ref64 = out.gather_columns(wide)
+    P = orc.phase_matrix(xyz, w, F, 1, 0.8, True)
+    ref_or = orc.scatter_matrix(illum.gather_cells(cells), P, threads=16)
+    log(f"[bench] config3 FP64 HG: {t64:.3f} ms")
+    for form, faces_note in (("bf16x2", "faces"), ("fp16x2", "faces"), ("bf16x3", "faces")):
+        sb, cb = rs.split_sizes(form, 4, n, m)
+        state = D.DeviceArray((sb,), np.uint8)
+        scale = D.DeviceArray((cb // 4,), np.float32)
+        rs.pack_split_dev(form, illum, 4, n, m, state, scale, stream=stream)
+        for _ in range(warmup):
+            ds.scatter_split(form, state, scale, 4, n, 0, m, out, stream=stream)
+        e0.record(stream)
+        for _ in range(steps):
+            ds.scatter_split(form, state, scale, 4, n, 0, m, out, stream=stream)
+        e1.record(stream)
+        t = e0.elapsed_ms(e1) / steps
+        stream.synchronize()
+        got_or = out.gather_columns(cells)
+        r_or = np.abs(got_or - ref_or) / ref_or
+        got = out.gather_columns(wide)
+        r = np.abs(got - ref64) / ref64
+        byts = sb + n * m * 8
+        res["forms"][form] = {
+            "ms": t, "speedup_vs_fp64": t64 / t, "pairs_per_s": n * m / (t * 1e-3),
+            "hbm_gbs_algorithmic": byts / (t * 1e-3) / 1e9, "roofline_frac_hbm": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_launch": byts, "state_bytes": sb,
+            "max_rel_err_vs_oracle": float(r_or.max()), "mean_rel_err_vs_oracle": float(r_or.mean()),
+            "max_rel_err_vs_fp64_gpu": float(r.max()), "mean_rel_err_vs_fp64_gpu": float(r.mean()),
+            "err_sample": f"{len(cells)} cells x {m} dirs vs oracle; {len(wide)} cells x {m} dirs vs FP64 GPU",
+        }
+        log(f"[bench] {form}: {t:.3f} ms ({t64 / t:.2f}x FP64), max rel err {r.max():.2e} mean {r.mean():.2e}")
+        state.free()
+        scale.free()
+    ds.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +133,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-slice", type=int, default=16)
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision (config 3) section")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="before the benchmark, run the plain copy kernel 3x over this many GB "
+                         "(reference for FETCH_SIZE / WRITE_SIZE counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -86,6 +146,17 @@ def main() -> int:
 
     from relativistic_solver_hllc_amd import device as D
     from relativistic_solver_hllc_amd import scatter as rs
+
+    if args.calibrate_copy > 0:
+        cnt = int(args.calibrate_copy * 1e9 / 8) // 2 * 2
+        a, b = D.DeviceArray((cnt,)), D.DeviceArray((cnt,))
+        a.fill(1.0)
+        for _ in range(3):
+            b.copy_kernel_from(a)
+        D.synchronize()
+        log(f"[bench] copy calibration: 3 x {cnt * 8} bytes read and written")
+        a.free()
+        b.free()
 
     n, m = args.cells, args.dirs
     xyz, w, F = rs.fibonacci_directions(m)
@@ -137,6 +208,10 @@ def main() -> int:
         if pmc.get("n_cells") == n and pmc.get("n_dirs") == m:
             traffic = pmc.get("hbm_bytes_per_launch")
 
+    split = None
+    if not args.no_split:
+        split = split_section(D, rs, illum, out, xyz, w, F, n, m, max(args.steps // 2, 2), 1, stream, cells)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -180,6 +255,7 @@ def main() -> int:
         },
         "parity": {"sampled_cells": int(len(cells)), "max_rel_err_vs_oracle": parity, "tolerance": 1e-12},
         "cpu_baseline": cpu,
+        "split": split,
     }
     ds.close()
     illum.free()

@@ -133,6 +133,24 @@ int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
                         int32_t k0, int32_t k1, double* d_out, int64_t ld_out, void* stream,
                         int32_t kernel, const rsh_tile* tile);
 
+/* Split-precision forms (RSH_FORM_BF16X2, RSH_FORM_FP16X2, RSH_FORM_BF16X3).
+ * No reference counterpart: the reference is FP64 only.
+ * The intensities are first packed once into a resident, MFMA-native split
+ * state (hi/lo [/mid] 16-bit parts per face, per-cell power-of-two scale),
+ * then the contraction reads only that state:
+ *   rsh_split_sizes        bytes of the state and of the per-cell scale array
+ *   rsh_pack_split_dev     FP64 illum [M][n_cells][faces] -> state + cell scales
+ *   rsh_scatter_split_dev  S rows [k0,k1) from the state, FP32 MFMA accumulation,
+ *                          written as FP64 (out_f32 = 0) or FP32 (out_f32 = 1)
+ *                          to d_out[(k - k0) * ld_out + c]                       */
+int rsh_split_sizes(int32_t form, int32_t faces, int64_t n_cells, int32_t m, int64_t* state_bytes,
+                    int64_t* scale_bytes);
+int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64_t n_cells, int32_t m,
+                       void* d_state, float* d_cell_scale, void* stream);
+int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, const float* d_cell_scale,
+                          int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
+                          int32_t out_f32, void* stream);
+
 /* Seeded log-normal fill exp(mu + sigma * N(0,1)) on the device (counter-based
  * generator: the value at index i depends only on (seed, i)). */
 int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed,
@@ -158,6 +176,9 @@ int rsh_dev_memcpy2d(void* dst, int64_t dpitch, const void* src, int64_t spitch,
                      int64_t height, int32_t kind);
 int rsh_dev_fill_f64(double* d, int64_t count, double value, void* stream);
 int rsh_dev_scale_f64(double* d, int64_t count, double factor, void* stream);
+/* Plain streaming copy kernel (16-byte loads/stores); reads and writes
+ * count*8 bytes, used to calibrate HBM byte counters. */
+int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream);
 int rsh_dev_synchronize(int32_t device);
 int rsh_dev_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
 int rsh_stream_create(int32_t device, void** out);

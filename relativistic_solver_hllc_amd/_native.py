@@ -59,6 +59,9 @@ PROTOTYPES = {
     "rsh_dirset_destroy": (C.c_int, [vp]),
     "rsh_dirset_phase_matrix": (C.c_int, [vp, i32, i32, vp]),
     "rsh_scatter_f64_dev": (C.c_int, [vp, vp, i32, i64, i32, i32, vp, i64, vp, i32, C.POINTER(Tile)]),
+    "rsh_split_sizes": (C.c_int, [i32, i32, i64, i32, C.POINTER(i64), C.POINTER(i64)]),
+    "rsh_pack_split_dev": (C.c_int, [i32, vp, i32, i64, i32, vp, vp, vp]),
+    "rsh_scatter_split_dev": (C.c_int, [vp, i32, vp, vp, i32, i64, i32, i32, vp, i64, i32, vp]),
     "rsh_fill_lognormal_dev": (C.c_int, [vp, i64, f64, f64, u64, vp]),
     "rsh_selftest_mfma_f64": (C.c_int, [i32, P_i32]),
     "rsh_dev_alloc": (C.c_int, [i32, i64, P_vp]),
@@ -69,6 +72,7 @@ PROTOTYPES = {
     "rsh_dev_memcpy2d": (C.c_int, [vp, i64, vp, i64, i64, i64, i32]),
     "rsh_dev_fill_f64": (C.c_int, [vp, i64, f64, vp]),
     "rsh_dev_scale_f64": (C.c_int, [vp, i64, f64, vp]),
+    "rsh_dev_copy_f64": (C.c_int, [vp, vp, i64, vp]),
     "rsh_dev_synchronize": (C.c_int, [i32]),
     "rsh_dev_mem_info": (C.c_int, [i32, C.POINTER(i64), C.POINTER(i64)]),
     "rsh_stream_create": (C.c_int, [i32, P_vp]),

@@ -29,8 +29,14 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{REPO_DIR}/include", f"-I{CSRC}"]
 
-HIP_SOURCES = ["kernels_f64.hip"]
+HIP_SOURCES = ["kernels_f64.hip", "kernels_split.hip"]
 HOST_SOURCES = ["capi.cpp"]
+
+# Drop-in shim of the reference's cuda_solve.h names (stand-alone types) and the
+# test driver that replays the reference's call sequence through it.
+SHIM_SRC = os.path.join(PKG_DIR, "shim", "cuda_solve_shim.cpp")
+SHIM_DRIVER_SRC = os.path.join(REPO_DIR, "tests", "shim", "shim_driver.cpp")
+SHIM_DRIVER = os.path.join(OBJ_DIR, "shim_driver")
 
 
 def _run(cmd):
@@ -63,8 +69,10 @@ def build(verbose: bool = False) -> str:
     tmp = LIB_PATH + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB_PATH)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{REPO_DIR}/include", SHIM_SRC, SHIM_DRIVER_SRC,
+          f"-L{PKG_DIR}", "-lrsh_scatter", f"-Wl,-rpath,{PKG_DIR}", "-Wl,-rpath,$ORIGIN/..", "-o", SHIM_DRIVER])
     if verbose:
-        print("built", LIB_PATH)
+        print("built", LIB_PATH, "and", SHIM_DRIVER)
     return LIB_PATH
 
 

@@ -44,8 +44,10 @@ using namespace rsh;
 // ------------------------------------------------------------------ objects
 
 struct PanelEntry {
-    double* d_panel = nullptr;
+    void* d_panel = nullptr;
+    float* d_row_inv = nullptr;  // split forms: per-row inverse scale
     F64Plan plan{};
+    SplitPlan splan{};
     hipEvent_t ready = nullptr;  // recorded after the panel build
 };
 
@@ -61,8 +63,8 @@ struct rsh_dirset_s {
     double* d_w = nullptr;
     double* d_P = nullptr;  // [m][m] FP64 phase matrix
     std::mutex mu;
-    // key: (k0, k1, faces)
-    std::map<std::tuple<int, int, int>, PanelEntry> panels;
+    // key: (k0, k1, faces, form)
+    std::map<std::tuple<int, int, int, int>, PanelEntry> panels;
     bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
 };
 
@@ -93,10 +95,16 @@ int check_device(int dev) {
     return RSH_OK;
 }
 
-// Panel for rows [k0,k1) and the given face count, built on stream st if absent.
-int get_panel(rsh_dirset ds, int k0, int k1, int faces, hipStream_t st, const PanelEntry** out) {
+void free_panel(PanelEntry& pe) {
+    hipFree(pe.d_panel);
+    hipFree(pe.d_row_inv);
+    if (pe.ready) hipEventDestroy(pe.ready);
+}
+
+// Panel for rows [k0,k1), face count and form, built on stream st if absent.
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out) {
     std::lock_guard<std::mutex> lock(ds->mu);
-    auto key = std::make_tuple(k0, k1, faces);
+    auto key = std::make_tuple(k0, k1, faces, form);
     auto it = ds->panels.find(key);
     if (it != ds->panels.end()) {
         // Make st wait for the build only while it may still be running.
@@ -108,23 +116,31 @@ int get_panel(rsh_dirset ds, int k0, int k1, int faces, hipStream_t st, const Pa
     if (ds->panels.size() >= 64) {
         // Bounded cache: drop everything after the device is idle.
         RSH_HIP_TRY(hipDeviceSynchronize());
-        for (auto& kv : ds->panels) {
-            hipFree(kv.second.d_panel);
-            hipEventDestroy(kv.second.ready);
-        }
+        for (auto& kv : ds->panels) free_panel(kv.second);
         ds->panels.clear();
     }
     PanelEntry pe;
-    pe.plan = plan_f64(ds->m, k1 - k0);
-    const int64_t nd = panel_f64_doubles(pe.plan);
-    RSH_HIP_TRY(hipMalloc(&pe.d_panel, std::max<int64_t>(nd, 1) * sizeof(double)));
-    hipError_t e = launch_panel_f64(ds->d_P, ds->m, k0, k1, 1.0 / faces, pe.plan, pe.d_panel, st);
-    if (e != hipSuccess) {
-        hipFree(pe.d_panel);
-        return hip_fail(e, "panel_f64_kernel");
+    hipError_t e = hipSuccess;
+    if (form == RSH_FORM_FP64) {
+        pe.plan = plan_f64(ds->m, k1 - k0);
+        const int64_t nd = panel_f64_doubles(pe.plan);
+        e = hipMalloc(&pe.d_panel, std::max<int64_t>(nd, 1) * sizeof(double));
+        if (e == hipSuccess)
+            e = launch_panel_f64(ds->d_P, ds->m, k0, k1, 1.0 / faces, pe.plan, static_cast<double*>(pe.d_panel), st);
+    } else {
+        pe.splan = plan_split(ds->m, k1 - k0);
+        const int64_t units = split_panel_units(form, pe.splan);
+        e = hipMalloc(&pe.d_panel, std::max<int64_t>(units, 1) * 16);
+        if (e == hipSuccess) e = hipMalloc(&pe.d_row_inv, std::max(pe.splan.rb_total * 16, 1) * sizeof(float));
+        if (e == hipSuccess)
+            e = launch_split_panel(form, ds->d_P, ds->m, k0, k1, 1.0 / faces, pe.splan, pe.d_panel, pe.d_row_inv, st);
     }
-    RSH_HIP_TRY(hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming));
-    RSH_HIP_TRY(hipEventRecord(pe.ready, st));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(pe.ready, st);
+    if (e != hipSuccess) {
+        free_panel(pe);
+        return hip_fail(e, "phase panel build");
+    }
     auto res = ds->panels.emplace(key, pe);
     *out = &res.first->second;
     return RSH_OK;
@@ -151,10 +167,32 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int 
     if (!mfma_f64_tile_supported(t.waves, t.chunk_steps))
         return fail(RSH_E_UNSUPPORTED, "unsupported MFMA tile (waves, chunk_steps)");
     const PanelEntry* pe = nullptr;
-    int rc = get_panel(ds, k0, k1, faces, st, &pe);
+    int rc = get_panel(ds, k0, k1, faces, RSH_FORM_FP64, st, &pe);
     if (rc) return rc;
-    RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ds->m, pe->d_panel, pe->plan, k1 - k0, d_out, ld, t.waves,
-                                t.chunk_steps, st));
+    RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ds->m, static_cast<const double*>(pe->d_panel), pe->plan, k1 - k0,
+                                d_out, ld, t.waves, t.chunk_steps, st));
+    return RSH_OK;
+}
+
+bool valid_split_form(int form) {
+    return form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3;
+}
+
+int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_cell_inv, int faces, int64_t n,
+                  int k0, int k1, void* d_out, int64_t ld, int out_f32, hipStream_t st) {
+    if (!ds) return fail(RSH_E_ARG, "null direction set");
+    if (!valid_split_form(form)) return fail(RSH_E_ARG, "not a split form");
+    if (faces != 1 && faces != 4) return fail(RSH_E_UNSUPPORTED, "faces must be 1 or 4");
+    if (n < 0 || k0 < 0 || k1 > ds->m || k0 > k1) return fail(RSH_E_ARG, "invalid cell count or direction range");
+    if (n == 0 || k0 == k1) return RSH_OK;
+    if (!d_state || !d_cell_inv || !d_out) return fail(RSH_E_ARG, "null device pointer");
+    if (ld < n) return fail(RSH_E_ARG, "ld_out smaller than n_cells");
+    RSH_HIP_TRY(hipSetDevice(ds->device));
+    const PanelEntry* pe = nullptr;
+    int rc = get_panel(ds, k0, k1, faces, form, st, &pe);
+    if (rc) return rc;
+    RSH_HIP_TRY(launch_split_mfma(form, d_state, faces, n, d_cell_inv, pe->d_panel, pe->d_row_inv, pe->splan,
+                                  k1 - k0, d_out, ld, out_f32 ? 1 : 0, st));
     return RSH_OK;
 }
 
@@ -292,10 +330,7 @@ int rsh_dirset_destroy(rsh_dirset ds) {
     if (!ds) return RSH_OK;
     hipSetDevice(ds->device);
     hipDeviceSynchronize();
-    for (auto& kv : ds->panels) {
-        hipFree(kv.second.d_panel);
-        hipEventDestroy(kv.second.ready);
-    }
+    for (auto& kv : ds->panels) free_panel(kv.second);
     hipFree(ds->d_dirs);
     hipFree(ds->d_w);
     hipFree(ds->d_P);
@@ -315,6 +350,35 @@ int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
                         int32_t k1, double* d_out, int64_t ld_out, void* stream, int32_t kernel,
                         const rsh_tile* tile) {
     return scatter_f64(ds, d_illum, faces, n_cells, k0, k1, d_out, ld_out, (hipStream_t)stream, kernel, tile);
+}
+
+int rsh_split_sizes(int32_t form, int32_t faces, int64_t n_cells, int32_t m, int64_t* state_bytes,
+                    int64_t* scale_bytes) {
+    if (!valid_split_form(form) || (faces != 1 && faces != 4) || n_cells < 0 || m <= 0 || !state_bytes ||
+        !scale_bytes)
+        return fail(RSH_E_ARG, "invalid split size arguments");
+    *state_bytes = split_state_units(form, faces, n_cells, m) * 16;
+    *scale_bytes = n_cells * (int64_t)sizeof(float);
+    return RSH_OK;
+}
+
+int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64_t n_cells, int32_t m,
+                       void* d_state, float* d_cell_scale, void* stream) {
+    if (!valid_split_form(form) || (faces != 1 && faces != 4) || n_cells < 0 || m <= 0)
+        return fail(RSH_E_ARG, "invalid split pack arguments");
+    if (n_cells == 0) return RSH_OK;
+    if (!d_illum || !d_state || !d_cell_scale) return fail(RSH_E_ARG, "null device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    RSH_HIP_TRY(launch_cell_scale(d_illum, n_cells, m, faces, d_cell_scale, st));
+    RSH_HIP_TRY(launch_pack_state(form, d_illum, faces, n_cells, m, d_cell_scale, d_state, st));
+    return RSH_OK;
+}
+
+int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, const float* d_cell_scale,
+                          int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
+                          int32_t out_f32, void* stream) {
+    return scatter_split(ds, form, d_state, d_cell_scale, faces, n_cells, k0, k1, d_out, ld_out, out_f32,
+                         (hipStream_t)stream);
 }
 
 int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed, void* stream) {
@@ -425,6 +489,13 @@ int rsh_dev_fill_f64(double* d, int64_t count, double value, void* stream) {
 int rsh_dev_scale_f64(double* d, int64_t count, double factor, void* stream) {
     if (count < 0 || (count > 0 && !d)) return fail(RSH_E_ARG, "invalid scale arguments");
     RSH_HIP_TRY(launch_scale_f64(d, count, factor, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream) {
+    if (count < 0 || (count > 0 && (!dst || !src))) return fail(RSH_E_ARG, "invalid copy arguments");
+    if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(RSH_E_ARG, "copy kernel needs 16-byte aligned pointers");
+    RSH_HIP_TRY(launch_copy_f64(dst, src, count, (hipStream_t)stream));
     return RSH_OK;
 }
 

@@ -405,6 +405,15 @@ __global__ __launch_bounds__(256) void scale_f64_kernel(double* __restrict__ d, 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) d[i] *= factor;
 }
 
+__global__ __launch_bounds__(256) void copy_f64_kernel(v2d* __restrict__ dst, const v2d* __restrict__ src,
+                                                       int64_t pairs) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+__global__ void copy_tail_kernel(double* __restrict__ dst, const double* __restrict__ src) { dst[0] = src[0]; }
+
 static unsigned grid_for(int64_t count) {
     int64_t b = (count + 255) / 256;
     return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
@@ -419,6 +428,20 @@ hipError_t launch_fill_f64(double* d, int64_t count, double value, hipStream_t s
 hipError_t launch_scale_f64(double* d, int64_t count, double factor, hipStream_t st) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(scale_f64_kernel, dim3(grid_for(count)), dim3(256), 0, st, d, count, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f64(double* dst, const double* src, int64_t count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const int64_t pairs = count / 2;
+    if (pairs > 0) {
+        int64_t b = (pairs + 255) / 256;
+        if (b > 8192) b = 8192;
+        hipLaunchKernelGGL(copy_f64_kernel, dim3((unsigned)b), dim3(256), 0, st, reinterpret_cast<v2d*>(dst),
+                           reinterpret_cast<const v2d*>(src), pairs);
+    }
+    if (count & 1)
+        hipLaunchKernelGGL(copy_tail_kernel, dim3(1), dim3(1), 0, st, dst + count - 1, src + count - 1);
     return hipGetLastError();
 }
 

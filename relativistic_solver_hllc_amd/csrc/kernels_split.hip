@@ -1,0 +1,406 @@
+// Split-precision scattering integral on CDNA4 (gfx950).
+//
+// Operands are stored as sums of 16-bit parts, x ~ x_hi + x_lo (+ x_mid for the
+// three-term BF16 form), and contracted on v_mfma_f32_16x16x32_{bf16,f16} with
+// FP32 accumulation:
+//     acc_main  += A_hi * B_hi
+//     acc_cross += A_hi * B_lo + A_lo * B_hi      (+ hi*lo, lo*hi, mid*mid for x3)
+// The cross terms are kept in their own accumulator so that their rounding is
+// relative to their own (2^-8 smaller) magnitude; the epilogue adds the two in
+// FP64 and removes the power-of-two operand scales exactly.
+//
+// Scaling: every cell c carries 2^e_c and every output row k carries 2^r_k,
+// chosen so that the largest face value of the cell / phase entry of the row
+// lands near 2^12. This keeps FP16 hi/lo parts inside the normal range for
+// log-normal intensities; for BF16 it is harmless (exact).
+//
+// Lane layouts (gfx950, measured with one-hot operands):
+//   A (16x32): lane l, element e -> A[l % 16][8 (l / 16) + e]
+//   B (32x16): lane l, element e -> B[8 (l / 16) + e][l % 16]
+//   D (16x16): lane l, item r    -> D[4 (l / 16) + r][l % 16]
+//
+// Packed intensity state ("split state"), MFMA-native:
+//   state[((ct * steps + s) * V + v) * 64 + lane] = 16 bytes (8 parts of 16 bits)
+//   ct = cell tile (16 cells), s = K-step (32 directions), v = part * F + face,
+//   lane l, element e <-> cell 16 ct + l % 16, direction 32 s + 8 (l / 16) + e.
+// One (ct, s, v) vector is 1 KB contiguous: a single coalesced wave load.
+//
+// Phase panel (A operand), per pass of RB row blocks:
+//   panel[(((pass * steps + s) * RB + rb) * NT + t) * 64 + lane] = 16 bytes
+//   lane l, element e <-> row k0 + 16 (pass RB + rb) + l % 16, direction 32 s + 8 (l / 16) + e.
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+constexpr int kTargetExp = 12;  // largest scaled magnitude ~ 2^12
+
+template <int FMT>
+struct Fmt;
+template <>
+struct Fmt<0> {  // BF16
+    typedef __bf16 T;
+    typedef v8bf V;
+    static __device__ inline v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Fmt<1> {  // FP16
+    typedef _Float16 T;
+    typedef v8h V;
+    static __device__ inline v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+// Splits x into NT parts of format T with round-to-nearest at each stage;
+// the residual subtraction is exact in FP32.
+template <int FMT, int NT>
+__device__ inline void split_parts(float x, typename Fmt<FMT>::T* parts) {
+    typedef typename Fmt<FMT>::T T;
+    float r = x;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const T p = (T)r;
+        parts[t] = p;
+        r = r - (float)p;
+    }
+}
+
+// ------------------------------------------------------------------ cell scales
+// inv_scale[c] = 2^-e_c with e_c = kTargetExp - 1 - ilogb(max_{j,f} |I[j][c][f]|).
+__global__ __launch_bounds__(256) void cell_scale_kernel(const double* __restrict__ illum, int64_t n, int m,
+                                                         int faces, float* __restrict__ inv_scale) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double mx = 0.0;
+    const double* p = illum + c * faces;
+    const int64_t stride = n * faces;
+    for (int j = 0; j < m; j++) {
+        for (int f = 0; f < faces; f++) mx = fmax(mx, fabs(p[f]));
+        p += stride;
+    }
+    const int e = mx > 0.0 ? (kTargetExp - 1 - ilogb(mx)) : 0;
+    inv_scale[c] = ldexpf(1.0f, -e);
+}
+
+// ------------------------------------------------------------------ state pack
+// Thread = (ct, s, lane). Reads 8 directions x F faces of one cell, writes NT*F vectors.
+template <int FMT, int NT, int F>
+__global__ __launch_bounds__(256) void pack_state_kernel(const double* __restrict__ illum, int64_t n, int m,
+                                                         int steps, const float* __restrict__ inv_scale,
+                                                         v4u* __restrict__ state, int64_t total) {
+    typedef typename Fmt<FMT>::T T;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx & 63);
+    const int64_t frag = idx >> 6;
+    const int s = (int)(frag % steps);
+    const int64_t ct = frag / steps;
+    const int64_t cell = ct * 16 + (lane & 15);
+    const int j0 = 32 * s + 8 * (lane >> 4);
+    const bool cvalid = cell < n;
+    const double scale = cvalid ? 1.0 / (double)inv_scale[cell] : 0.0;
+    T buf[NT][F][8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int j = j0 + e;
+        const bool ok = cvalid && j < m;
+        const double* p = illum + ((int64_t)(ok ? j : 0) * n + (ok ? cell : 0)) * F;
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+            const float x = ok ? (float)(p[f] * scale) : 0.0f;
+            T parts[NT];
+            split_parts<FMT, NT>(x, parts);
+#pragma unroll
+            for (int t = 0; t < NT; t++) buf[t][f][e] = parts[t];
+        }
+    }
+    v4u* out = state + (frag * (NT * F)) * 64 + lane;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int f = 0; f < F; f++) out[(t * F + f) * 64] = __builtin_bit_cast(v4u, buf[t][f]);
+}
+
+// ------------------------------------------------------------------ phase panel
+// One block per output row: row scale from the row maximum, then the fragments.
+template <int FMT, int NT>
+__global__ __launch_bounds__(256) void split_panel_kernel(const double* __restrict__ P, int m, int k0, int k1,
+                                                          double factor, int rb_per_pass, int steps,
+                                                          v4u* __restrict__ panel, float* __restrict__ row_inv) {
+    typedef typename Fmt<FMT>::T T;
+    const int r = blockIdx.x;  // local row, padded range [0, passes * RB * 16)
+    const int k = k0 + r;
+    const bool valid = k < k1;
+    __shared__ double red[256];
+    double mx = 0.0;
+    if (valid)
+        for (int j = threadIdx.x; j < m; j += blockDim.x) mx = fmax(mx, fabs(P[(int64_t)k * m + j] * factor));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    const int e = red[0] > 0.0 ? (kTargetExp - 1 - ilogb(red[0])) : 0;
+    const double scale = ldexp(1.0, e);
+    if (threadIdx.x == 0) row_inv[r] = ldexpf(1.0f, -e);
+    const int rb_g = r / 16, rl = r % 16;
+    const int pass = rb_g / rb_per_pass, rb = rb_g % rb_per_pass;
+    // each (s, lane-group g) slot covers directions 32 s + 8 g + e
+    for (int slot = threadIdx.x; slot < steps * 4; slot += blockDim.x) {
+        const int s = slot >> 2, g = slot & 3;
+        T buf[NT][8];
+#pragma unroll
+        for (int e8 = 0; e8 < 8; e8++) {
+            const int j = 32 * s + 8 * g + e8;
+            const float x = (valid && j < m) ? (float)(P[(int64_t)k * m + j] * factor * scale) : 0.0f;
+            T parts[NT];
+            split_parts<FMT, NT>(x, parts);
+#pragma unroll
+            for (int t = 0; t < NT; t++) buf[t][e8] = parts[t];
+        }
+        const int lane = g * 16 + rl;
+        v4u* out = panel + ((((int64_t)pass * steps + s) * rb_per_pass + rb) * NT) * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < NT; t++) out[t * 64] = __builtin_bit_cast(v4u, buf[t]);
+    }
+}
+
+// ------------------------------------------------------------------ contraction
+template <int RB, int W, int FMT, int NT, int F>
+__global__ __launch_bounds__(64 * W) void split_mfma_kernel(const v4u* __restrict__ state, int64_t n, int steps,
+                                                            const float* __restrict__ cell_inv,
+                                                            const v4u* __restrict__ panel,
+                                                            const float* __restrict__ row_inv, void* __restrict__ out,
+                                                            int64_t ld, int rows, int out_f32) {
+    typedef typename Fmt<FMT>::T T;
+    typedef typename Fmt<FMT>::V V;
+    constexpr int UNITS = RB * NT * 64;  // 16-byte units per LDS stage (one K-step)
+    constexpr int THREADS = 64 * W;
+    constexpr int PER = (UNITS + THREADS - 1) / THREADS;
+    constexpr bool TAIL = (UNITS % THREADS) != 0;
+    constexpr int VX = NT * F;  // state vectors per (cell tile, step)
+    __shared__ v4u lds[2][UNITS];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t ct = (int64_t)blockIdx.x * W + wave;
+    const int64_t cell = ct * 16 + (lane & 15);
+    const bool tvalid = ct * 16 < n;
+    const int rb0 = blockIdx.y * RB;
+
+    const v4u* psrc = panel + (int64_t)blockIdx.y * steps * UNITS + tid;
+    const v4u* xsrc = state + (tvalid ? ct : 0) * (int64_t)steps * VX * 64 + lane;
+
+    v4f accm[RB], accx[RB];
+#pragma unroll
+    for (int i = 0; i < RB; i++) {
+        accm[i] = v4f{0.f, 0.f, 0.f, 0.f};
+        accx[i] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    v4u preg[PER];
+    v4u xr[VX];
+    V bx[NT];
+
+#define RSH_LOAD_PANEL(t)                                                    \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
+            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];              \
+    }
+#define RSH_STORE_PANEL(buf)                                                 \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
+            lds[buf][tid + i * THREADS] = preg[i];                           \
+    }
+#define RSH_LOAD_X(t)                                                        \
+    if (tvalid) {                                                            \
+        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = xsrc[((int64_t)(t) * VX + v) * 64]; \
+    } else {                                                                 \
+        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = v4u{0u, 0u, 0u, 0u}; \
+    }
+
+    RSH_LOAD_PANEL(0)
+    RSH_LOAD_X(0)
+    RSH_STORE_PANEL(0)
+    __syncthreads();
+
+    for (int t = 0; t < steps; t++) {
+        const int buf = t & 1;
+        const bool more = t + 1 < steps;
+        // B operand of this step from the loaded state vectors
+        if constexpr (F == 1) {
+#pragma unroll
+            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, xr[p]);
+        } else {
+            T parts[NT][8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                float x = 0.0f;
+#pragma unroll
+                for (int f = 0; f < F; f++) {
+                    float fv = 0.0f;
+#pragma unroll
+                    for (int p = NT - 1; p >= 0; p--) fv += (float)__builtin_bit_cast(V, xr[p * F + f])[e];
+                    x += fv;
+                }
+                T q[NT];
+                split_parts<FMT, NT>(x, q);
+#pragma unroll
+                for (int p = 0; p < NT; p++) parts[p][e] = q[p];
+            }
+#pragma unroll
+            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, parts[p]);
+        }
+        if (more) {
+            RSH_LOAD_PANEL(t + 1)
+            RSH_LOAD_X(t + 1)
+        }
+        const v4u* a_base = lds[buf] + lane;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const V a0 = __builtin_bit_cast(V, a_base[(rb * NT + 0) * 64]);
+            const V a1 = __builtin_bit_cast(V, a_base[(rb * NT + 1) * 64]);
+            accm[rb] = Fmt<FMT>::mfma(a0, bx[0], accm[rb]);
+            if constexpr (NT == 2) {
+                accx[rb] = Fmt<FMT>::mfma(a0, bx[1], accx[rb]);
+                accx[rb] = Fmt<FMT>::mfma(a1, bx[0], accx[rb]);
+            } else {
+                const V a2 = __builtin_bit_cast(V, a_base[(rb * NT + 2) * 64]);
+                accx[rb] = Fmt<FMT>::mfma(a1, bx[1], accx[rb]);
+                accx[rb] = Fmt<FMT>::mfma(a0, bx[2], accx[rb]);
+                accx[rb] = Fmt<FMT>::mfma(a2, bx[0], accx[rb]);
+                accx[rb] = Fmt<FMT>::mfma(a0, bx[1], accx[rb]);
+                accx[rb] = Fmt<FMT>::mfma(a1, bx[0], accx[rb]);
+            }
+        }
+        if (more) RSH_STORE_PANEL(buf ^ 1)
+        __syncthreads();
+    }
+#undef RSH_LOAD_PANEL
+#undef RSH_STORE_PANEL
+#undef RSH_LOAD_X
+
+    if (!tvalid || cell >= n) return;
+    const double cinv = (double)cell_inv[cell];
+    const int g = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = (rb0 + rb) * 16 + 4 * g + r;
+            if (row < rows) {
+                const double v = ((double)accm[rb][r] + (double)accx[rb][r]) * (double)row_inv[row] * cinv;
+                if (out_f32)
+                    __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
+                else
+                    __builtin_nontemporal_store(v, static_cast<double*>(out) + (int64_t)row * ld + cell);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+SplitPlan plan_split(int m, int rows) {
+    SplitPlan pl{};
+    const F64Plan fp = plan_f64(m, rows);  // same row-block / pass selection
+    pl.rb_per_pass = fp.rb_per_pass;
+    pl.passes = fp.passes;
+    pl.rb_total = fp.rb_total;
+    pl.steps = (m + 31) / 32;
+    return pl;
+}
+
+int split_terms(int form) { return form == RSH_FORM_BF16X3 ? 3 : 2; }
+int split_fmt(int form) { return form == RSH_FORM_FP16X2 ? 1 : 0; }
+
+int64_t split_state_units(int form, int faces, int64_t n, int m) {
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t steps = (m + 31) / 32;
+    return tiles * steps * split_terms(form) * faces * 64;
+}
+
+int64_t split_panel_units(int form, const SplitPlan& pl) {
+    return (int64_t)pl.passes * pl.steps * pl.rb_per_pass * split_terms(form) * 64;
+}
+
+hipError_t launch_cell_scale(const double* d_illum, int64_t n, int m, int faces, float* d_inv, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cell_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_illum, n, m, faces,
+                       d_inv);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t n, int m, const float* d_inv,
+                             void* d_state, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int steps = (m + 31) / 32;
+    const int64_t total = (n + 15) / 16 * steps * 64;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    v4u* out = static_cast<v4u*>(d_state);
+#define RSH_PACK(FMT, NT, F) hipLaunchKernelGGL((pack_state_kernel<FMT, NT, F>), grid, block, 0, st, d_illum, n, m, steps, d_inv, out, total)
+    const int fmt = split_fmt(form), nt = split_terms(form);
+    if (fmt == 0 && nt == 2) { if (faces == 4) RSH_PACK(0, 2, 4); else RSH_PACK(0, 2, 1); }
+    else if (fmt == 1 && nt == 2) { if (faces == 4) RSH_PACK(1, 2, 4); else RSH_PACK(1, 2, 1); }
+    else { if (faces == 4) RSH_PACK(0, 3, 4); else RSH_PACK(0, 3, 1); }
+#undef RSH_PACK
+    return hipGetLastError();
+}
+
+hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor, const SplitPlan& pl,
+                              void* d_panel, float* d_row_inv, hipStream_t st) {
+    const int rows_padded = pl.rb_total * 16;
+    if (rows_padded == 0) return hipSuccess;
+    v4u* out = static_cast<v4u*>(d_panel);
+#define RSH_PANEL(FMT, NT) hipLaunchKernelGGL((split_panel_kernel<FMT, NT>), dim3(rows_padded), dim3(256), 0, st, d_P, m, k0, k1, factor, pl.rb_per_pass, pl.steps, out, d_row_inv)
+    const int fmt = split_fmt(form), nt = split_terms(form);
+    if (fmt == 0 && nt == 2) RSH_PANEL(0, 2);
+    else if (fmt == 1) RSH_PANEL(1, 2);
+    else RSH_PANEL(0, 3);
+#undef RSH_PANEL
+    return hipGetLastError();
+}
+
+template <int W, int FMT, int NT, int F>
+static hipError_t launch_split_rb(int rb, dim3 grid, hipStream_t st, const v4u* state, int64_t n, int steps,
+                                  const float* cell_inv, const v4u* panel, const float* row_inv, void* out,
+                                  int64_t ld, int rows, int out_f32) {
+#define RSH_RB_CASE(V)                                                                                          \
+    case V:                                                                                                     \
+        hipLaunchKernelGGL((split_mfma_kernel<V, W, FMT, NT, F>), grid, dim3(64 * W), 0, st, state, n, steps,  \
+                           cell_inv, panel, row_inv, out, ld, rows, out_f32);                                   \
+        break;
+    switch (rb) {
+        RSH_RB_CASE(1) RSH_RB_CASE(2) RSH_RB_CASE(3) RSH_RB_CASE(4) RSH_RB_CASE(5) RSH_RB_CASE(6)
+        RSH_RB_CASE(7) RSH_RB_CASE(8) RSH_RB_CASE(10) RSH_RB_CASE(13) RSH_RB_CASE(16) RSH_RB_CASE(20)
+        RSH_RB_CASE(25)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef RSH_RB_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n, const float* d_cell_inv,
+                             const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int rows, void* d_out,
+                             int64_t ld, int out_f32, hipStream_t st) {
+    if (n <= 0 || rows <= 0) return hipSuccess;
+    constexpr int W = 4;
+    const int64_t tiles = (n + 15) / 16;
+    const dim3 grid((unsigned)((tiles + W - 1) / W), (unsigned)pl.passes);
+    const v4u* state = static_cast<const v4u*>(d_state);
+    const v4u* panel = static_cast<const v4u*>(d_panel);
+    const int fmt = split_fmt(form), nt = split_terms(form), rb = pl.rb_per_pass;
+#define RSH_GO(FMT, NT, F) return launch_split_rb<W, FMT, NT, F>(rb, grid, st, state, n, pl.steps, d_cell_inv, panel, d_row_inv, d_out, ld, rows, out_f32)
+    if (fmt == 0 && nt == 2) { if (faces == 4) RSH_GO(0, 2, 4); RSH_GO(0, 2, 1); }
+    if (fmt == 1 && nt == 2) { if (faces == 4) RSH_GO(1, 2, 4); RSH_GO(1, 2, 1); }
+    if (faces == 4) RSH_GO(0, 3, 4);
+    RSH_GO(0, 3, 1);
+#undef RSH_GO
+}
+
+}  // namespace rsh

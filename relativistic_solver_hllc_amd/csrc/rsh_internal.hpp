@@ -57,11 +57,33 @@ hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, c
                            const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves,
                            int chunk_steps, hipStream_t st);
 
+// ------------------------------------------------------------------ split forms
+struct SplitPlan {
+    int rb_per_pass;
+    int passes;
+    int rb_total;
+    int steps;  // K-steps of 32 directions
+};
+SplitPlan plan_split(int m, int rows);
+int split_terms(int form);  // 2 or 3 parts per operand
+int split_fmt(int form);    // 0 = BF16, 1 = FP16
+int64_t split_state_units(int form, int faces, int64_t n, int m);  // 16-byte units
+int64_t split_panel_units(int form, const SplitPlan& pl);
+hipError_t launch_cell_scale(const double* d_illum, int64_t n, int m, int faces, float* d_inv, hipStream_t st);
+hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t n, int m, const float* d_inv,
+                             void* d_state, hipStream_t st);
+hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
+                              const SplitPlan& pl, void* d_panel, float* d_row_inv, hipStream_t st);
+hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n, const float* d_cell_inv,
+                             const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int rows,
+                             void* d_out, int64_t ld, int out_f32, hipStream_t st);
+
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
                                  hipStream_t st);
 
 hipError_t launch_fill_f64(double* d, int64_t count, double value, hipStream_t st);
 hipError_t launch_scale_f64(double* d, int64_t count, double factor, hipStream_t st);
+hipError_t launch_copy_f64(double* dst, const double* src, int64_t count, hipStream_t st);
 
 hipError_t launch_mfma_layout_probe(const double* d_a, const double* d_b, double* d_d, hipStream_t st);
 

@@ -66,6 +66,11 @@ class DeviceArray:
         assert self.dtype == np.float64
         N.call("rsh_dev_scale_f64", self.ptr, self.size, float(factor), _stream_ptr(stream))
 
+    def copy_kernel_from(self, src: "DeviceArray", stream=None) -> None:
+        """Device copy through the library's streaming copy kernel (not an SDMA copy)."""
+        assert src.nbytes == self.nbytes and self.dtype == np.float64
+        N.call("rsh_dev_copy_f64", self.ptr, src.ptr, self.size, _stream_ptr(stream))
+
     def gather_cells(self, cells: Sequence[int]) -> np.ndarray:
         """For a [M][N][F] array: host copy of [M][len(cells)][F] (one 2D copy per cell)."""
         M, Nc, F = self.shape

@@ -14,7 +14,8 @@ reference's host API (solver_hllc_illum_cuda/cuda/cuda_solve.h:5-37):
     CudaEventPrint                              Problem.event_times
 
 Device memory handed to the *_dev functions is any object with a device
-address (torch CUDA tensors are used by the tests and the benchmark).
+address: relativistic_solver_hllc_amd.device.DeviceArray (the library's own
+allocations), or a raw integer address.
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ import numpy as np
 from . import _native as N
 
 PHASES = {"thomson": 0, "hg": 1}
+FORMS = {"fp64": 0, "bf16x2": 1, "fp16x2": 2, "bf16x3": 3}
 F64_KERNELS = {"auto": 0, "direct": 1, "mfma": 2}
 STREAM_SCATTERING_1, STREAM_SCATTERING_2, STREAM_PARAMS = 0, 1, 2
 
@@ -99,6 +101,19 @@ def fill_lognormal_dev(out, mu: float, sigma: float, seed: int, stream=None) -> 
            _stream(stream))
 
 
+def split_sizes(form: str, faces: int, n_cells: int, m: int) -> Tuple[int, int]:
+    """Bytes of the packed split state and of its per-cell scale array."""
+    a, b = C.c_int64(), C.c_int64()
+    N.call("rsh_split_sizes", FORMS[form], faces, n_cells, m, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def pack_split_dev(form: str, illum, faces: int, n_cells: int, m: int, state, cell_scale, stream=None) -> None:
+    """FP64 illum [M][n_cells][faces] -> resident split state + per-cell inverse scales."""
+    N.call("rsh_pack_split_dev", FORMS[form], _ptr(illum), faces, n_cells, m, _ptr(state), _ptr(cell_scale),
+           _stream(stream))
+
+
 # ------------------------------------------------------------------ direction set
 
 class DirectionSet:
@@ -127,6 +142,12 @@ class DirectionSet:
         t = C.byref(N.Tile(*tile)) if tile is not None else None
         N.call("rsh_scatter_f64_dev", self.handle, _ptr(illum), faces, n_cells, k0, k1, _ptr(out),
                n_cells if ld is None else ld, _stream(stream), F64_KERNELS[kernel], t)
+
+    def scatter_split(self, form: str, state, cell_scale, faces: int, n_cells: int, k0: int, k1: int, out,
+                      ld: Optional[int] = None, out_f32: bool = False, stream=None) -> None:
+        """Split-precision S rows [k0, k1) from a packed state (see pack_split_dev)."""
+        N.call("rsh_scatter_split_dev", self.handle, FORMS[form], _ptr(state), _ptr(cell_scale), faces, n_cells, k0,
+               k1, _ptr(out), n_cells if ld is None else ld, int(out_f32), _stream(stream))
 
     def close(self) -> None:
         if self.handle:
