@@ -156,6 +156,9 @@ int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, cons
 int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed,
                            void* stream);
 
+/* Seeded uniform fill lo + (hi - lo) * U[0,1) (same counter-based generator). */
+int rsh_fill_uniform_dev(double* d_out, int64_t count, double lo, double hi, uint64_t seed, void* stream);
+
 /* Checks the lane layout of v_mfma_f64_16x16x4f64 on the device against a
  * host product; *mismatches = number of differing outputs (0 expected). */
 int rsh_selftest_mfma_f64(int32_t device, int32_t* mismatches);
@@ -188,6 +191,43 @@ int rsh_event_create(void** out);
 int rsh_event_destroy(void* ev);
 int rsh_event_record(void* ev, void* stream);
 int rsh_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
+
+/* ------------------------------------------------ tile autotuner */
+/* Times every supported FP64 MFMA tile on a synthetic problem (n_cells cells,
+ * m directions, all m output rows) and records the fastest for the row-block
+ * class of m; later FP64 launches (RSH_F64_AUTO) and batches of that class use
+ * it. No reference counterpart. */
+int rsh_autotune_f64(int32_t device, int32_t m, int32_t faces, int64_t n_cells, rsh_tile* best, float* best_ms);
+int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
+
+/* ------------------------------------------------ batched source iteration */
+/* Source iteration I <- (1 - a) B + a S(I) on cell-level state [M][n_cells]
+ * for many uneven problems at once, with the residual of ReCalcIllum
+ * (solve_module/illum/Illum_async.cpp:472-510): max |I_old - I_new| / I_new, in
+ * FP64. The reference runs one problem per process and loops on the host
+ * (Illum_async.cpp:663-1060); here problems share persistent grouped launches
+ * (one per row-block class) captured as a hipGraph, the state stays resident,
+ * and problems join and leave between runs (continuous batching).
+ *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM)
+ *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot;
+ *                        ds holds the directions and phase matrix
+ *   rsh_batch_run        enqueues `iterations` iterations of every active slot
+ *                        on the batch's stream (asynchronous)
+ *   rsh_batch_residuals  per slot: residual of its last iteration (-1 if none)
+ *                        and iterations done (-1 = inactive); synchronises
+ *   rsh_batch_get_state  current state of a slot into d_out ([M][n_cells] FP64) */
+typedef struct rsh_batch_s* rsh_batch;
+int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes);
+int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_batch* out);
+int rsh_batch_destroy(rsh_batch b);
+int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B,
+                  double albedo, int32_t form, int32_t* slot);
+int rsh_batch_remove(rsh_batch b, int32_t slot);
+int rsh_batch_run(rsh_batch b, int32_t iterations);
+int rsh_batch_sync(rsh_batch b);
+int rsh_batch_residuals(rsh_batch b, double* resid, int32_t* iterations);
+int rsh_batch_get_state(rsh_batch b, int32_t slot, double* d_out);
+int rsh_batch_stream(rsh_batch b, void** stream);
 
 /* -------------------------------------------- reference-shaped problem API */
 

@@ -63,6 +63,19 @@ PROTOTYPES = {
     "rsh_pack_split_dev": (C.c_int, [i32, vp, i32, i64, i32, vp, vp, vp]),
     "rsh_scatter_split_dev": (C.c_int, [vp, i32, vp, vp, i32, i64, i32, i32, vp, i64, i32, vp]),
     "rsh_fill_lognormal_dev": (C.c_int, [vp, i64, f64, f64, u64, vp]),
+    "rsh_fill_uniform_dev": (C.c_int, [vp, i64, f64, f64, u64, vp]),
+    "rsh_autotune_f64": (C.c_int, [i32, i32, i32, i64, C.POINTER(Tile), P_f32]),
+    "rsh_tuned_tile_f64": (C.c_int, [i32, i32, C.POINTER(Tile)]),
+    "rsh_batch_footprint": (C.c_int, [i32, i64, i32, C.POINTER(i64)]),
+    "rsh_batch_create": (C.c_int, [i32, i32, i32, P_vp]),
+    "rsh_batch_destroy": (C.c_int, [vp]),
+    "rsh_batch_add": (C.c_int, [vp, vp, i64, vp, vp, f64, i32, P_i32]),
+    "rsh_batch_remove": (C.c_int, [vp, i32]),
+    "rsh_batch_run": (C.c_int, [vp, i32]),
+    "rsh_batch_sync": (C.c_int, [vp]),
+    "rsh_batch_residuals": (C.c_int, [vp, vp, vp]),
+    "rsh_batch_get_state": (C.c_int, [vp, i32, vp]),
+    "rsh_batch_stream": (C.c_int, [vp, P_vp]),
     "rsh_selftest_mfma_f64": (C.c_int, [i32, P_i32]),
     "rsh_dev_alloc": (C.c_int, [i32, i64, P_vp]),
     "rsh_dev_free": (C.c_int, [vp]),

@@ -1,0 +1,449 @@
+// Batches of uneven source-iteration problems (C ABI: rsh_batch_*), and the
+// FP64 tile autotuner (rsh_autotune_f64).
+//
+// A batch owns, per problem slot, the ping-pong cell-level state and the
+// source term; the phase panel comes from the problem's direction set. Slots
+// are grouped by row-block class (row blocks per pass of the MFMA kernel:
+// 13 for M = 200, 25 for M = 400 and M = 800); every class has a device work
+// list of (problem, pass, cell tile) items that one persistent launch consumes.
+// One batch iteration = a zeroing kernel + one grouped launch per class. Two
+// iterations (parity 0 then 1) are captured once into a hipGraph; changing the
+// active set rewrites the work lists and counts in device memory, which leaves
+// the captured graph valid (only a new class forces a re-capture).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "rsh_objects.hpp"
+
+using namespace rsh;
+
+namespace {
+
+constexpr int kMaxClasses = 16;
+
+struct Slot {
+    bool active = false;
+    rsh_dirset ds = nullptr;
+    int64_t n = 0;
+    int m = 0;
+    int form = RSH_FORM_FP64;
+    double albedo = 0;
+    double* I[2] = {nullptr, nullptr};
+    double* B = nullptr;
+    F64Plan plan{};
+    int iterations = 0;
+    int64_t bytes = 0;
+};
+
+struct ClassState {
+    int rb = 0;
+    rsh_tile tile{8, 2};
+    int grid = 0;
+    BatchItem* d_items = nullptr;
+    int64_t capacity = 0;
+    int* d_n_items = nullptr;
+    int n_items = 0;
+    int counter_index = 0;
+};
+
+}  // namespace
+
+struct rsh_batch_s {
+    int device = 0;
+    int capacity = 0;
+    bool use_graph = true;
+    std::vector<Slot> slots;
+    std::vector<BatchProblem> h_probs;
+    BatchProblem* d_probs = nullptr;
+    unsigned long long* d_resid = nullptr;  // 2 * capacity
+    unsigned* d_counters = nullptr;         // kMaxClasses
+    std::map<int, ClassState> classes;      // by row-block class
+    bool lists_dirty = true;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    int parity = 0;
+};
+
+namespace {
+
+void destroy_graph(rsh_batch b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+// Rebuilds the device descriptor table and every class's work list.
+int rebuild(rsh_batch b) {
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    std::map<int, std::vector<BatchItem>> lists;
+    for (auto& kv : b->classes) lists[kv.first];  // keep existing classes (possibly empty)
+    // Items of more expensive problems first (longest-processing-time order).
+    std::vector<int> order;
+    for (int i = 0; i < b->capacity; i++)
+        if (b->slots[i].active) order.push_back(i);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return b->slots[x].m > b->slots[y].m; });
+    bool new_class = false;
+    for (int i : order) {
+        const Slot& s = b->slots[i];
+        const int rb = s.plan.rb_per_pass;
+        if (!b->classes.count(rb)) {
+            if ((int)b->classes.size() >= kMaxClasses) return fail(RSH_E_UNSUPPORTED, "too many row-block classes");
+            ClassState cs;
+            cs.rb = rb;
+            cs.tile = f64_tile_for(b->device, rb);
+            cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
+            cs.counter_index = (int)b->classes.size();
+            RSH_HIP_TRY(hipMalloc(&cs.d_n_items, sizeof(int)));
+            b->classes[rb] = cs;
+            new_class = true;
+        }
+        const int tiles_w = 16 * b->classes[rb].tile.waves;
+        const int64_t tiles = (s.n + tiles_w - 1) / tiles_w;
+        auto& L = lists[rb];
+        for (int pass = 0; pass < s.plan.passes; pass++)
+            for (int64_t t = 0; t < tiles; t++) L.push_back(BatchItem{i, pass, t});
+    }
+    for (auto& kv : b->classes) {
+        ClassState& cs = kv.second;
+        auto& L = lists[kv.first];
+        if ((int64_t)L.size() > cs.capacity) {
+            hipFree(cs.d_items);
+            cs.d_items = nullptr;
+            cs.capacity = std::max<int64_t>((int64_t)L.size(), 1024);
+            RSH_HIP_TRY(hipMalloc(&cs.d_items, cs.capacity * sizeof(BatchItem)));
+            new_class = true;  // kernel argument (items pointer) changed
+        }
+        if (!L.empty())
+            RSH_HIP_TRY(hipMemcpy(cs.d_items, L.data(), L.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+        cs.n_items = (int)L.size();
+        RSH_HIP_TRY(hipMemcpy(cs.d_n_items, &cs.n_items, sizeof(int), hipMemcpyHostToDevice));
+    }
+    RSH_HIP_TRY(hipMemcpy(b->d_probs, b->h_probs.data(), b->capacity * sizeof(BatchProblem), hipMemcpyHostToDevice));
+    if (new_class) destroy_graph(b);
+    b->lists_dirty = false;
+    return RSH_OK;
+}
+
+int enqueue_iteration(rsh_batch b, int parity) {
+    RSH_HIP_TRY(launch_batch_zero(b->d_probs, b->capacity, b->d_counters, kMaxClasses, parity, b->stream));
+    for (auto& kv : b->classes) {
+        ClassState& cs = kv.second;
+        RSH_HIP_TRY(launch_grouped_iter_f64(cs.rb, cs.tile.waves, cs.tile.chunk_steps, cs.grid, b->d_probs,
+                                            cs.d_items, cs.d_n_items, b->d_counters + cs.counter_index, parity,
+                                            b->stream));
+    }
+    return RSH_OK;
+}
+
+int capture(rsh_batch b) {
+    destroy_graph(b);
+    RSH_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iteration(b, 0);
+    if (rc == RSH_OK) rc = enqueue_iteration(b, 1);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != RSH_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    RSH_HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->graph_valid = true;
+    return RSH_OK;
+}
+
+int64_t footprint(int form, int64_t n, int m) {
+    if (form == RSH_FORM_FP64) return 3 * (int64_t)m * n * 8;  // I ping-pong + B
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes) {
+    if (!bytes || n_cells < 0 || m <= 0) return fail(RSH_E_ARG, "invalid footprint arguments");
+    const int64_t f = footprint(form, n_cells, m);
+    if (f < 0) return fail(RSH_E_UNSUPPORTED, "form not supported by batches");
+    *bytes = f;
+    return RSH_OK;
+}
+
+int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_batch* out) {
+    if (!out || capacity <= 0) return fail(RSH_E_ARG, "invalid batch arguments");
+    *out = nullptr;
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    auto b = std::make_unique<rsh_batch_s>();
+    b->device = device;
+    b->capacity = capacity;
+    b->use_graph = use_graph != 0;
+    b->slots.resize(capacity);
+    b->h_probs.assign(capacity, BatchProblem{});
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_probs, capacity * sizeof(BatchProblem))) != hipSuccess ||
+        (e = hipMalloc(&b->d_resid, 2 * (size_t)capacity * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&b->d_counters, kMaxClasses * sizeof(unsigned))) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+        hipFree(b->d_probs);
+        hipFree(b->d_resid);
+        hipFree(b->d_counters);
+        return hip_fail(e, "batch allocation");
+    }
+    for (int i = 0; i < capacity; i++) b->h_probs[i].resid = b->d_resid + 2 * i;
+    RSH_HIP_TRY(hipMemcpy(b->d_probs, b->h_probs.data(), capacity * sizeof(BatchProblem), hipMemcpyHostToDevice));
+    *out = b.release();
+    return RSH_OK;
+}
+
+int rsh_batch_destroy(rsh_batch b) {
+    if (!b) return RSH_OK;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    destroy_graph(b);
+    for (auto& s : b->slots) {
+        hipFree(s.I[0]);
+        hipFree(s.I[1]);
+        hipFree(s.B);
+    }
+    for (auto& kv : b->classes) {
+        hipFree(kv.second.d_items);
+        hipFree(kv.second.d_n_items);
+    }
+    hipFree(b->d_probs);
+    hipFree(b->d_resid);
+    hipFree(b->d_counters);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return RSH_OK;
+}
+
+int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B, double albedo,
+                  int32_t form, int32_t* slot) {
+    if (!b || !ds || !slot || n_cells <= 0 || !d_I0 || !d_B) return fail(RSH_E_ARG, "invalid batch_add arguments");
+    if (form != RSH_FORM_FP64) return fail(RSH_E_UNSUPPORTED, "batches run the FP64 form");
+    if (ds->device != b->device) return fail(RSH_E_ARG, "direction set lives on another device");
+    int id = -1;
+    for (int i = 0; i < b->capacity; i++)
+        if (!b->slots[i].active) {
+            id = i;
+            break;
+        }
+    if (id < 0) return fail(RSH_E_NOMEM, "batch is full");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    Slot s;
+    s.ds = ds;
+    s.n = n_cells;
+    s.m = ds->m;
+    s.form = form;
+    s.albedo = albedo;
+    s.plan = plan_f64(ds->m, ds->m);
+    const size_t bytes = (size_t)ds->m * n_cells * sizeof(double);
+    hipError_t e;
+    if ((e = hipMalloc(&s.I[0], bytes)) != hipSuccess || (e = hipMalloc(&s.I[1], bytes)) != hipSuccess ||
+        (e = hipMalloc(&s.B, bytes)) != hipSuccess) {
+        hipFree(s.I[0]);
+        hipFree(s.I[1]);
+        hipFree(s.B);
+        return hip_fail(e, "batch slot allocation");
+    }
+    s.bytes = 3 * (int64_t)bytes;
+    // The next iteration reads I[parity].
+    RSH_HIP_TRY(hipMemcpyAsync(s.I[b->parity], d_I0, bytes, hipMemcpyDeviceToDevice, b->stream));
+    RSH_HIP_TRY(hipMemcpyAsync(s.B, d_B, bytes, hipMemcpyDeviceToDevice, b->stream));
+    const PanelEntry* pe = nullptr;
+    int rc = get_panel(ds, 0, ds->m, 1, RSH_FORM_FP64, b->stream, &pe);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    s.active = true;
+    BatchProblem& P = b->h_probs[id];
+    P.panel = static_cast<const double*>(pe->d_panel);
+    P.I[0] = s.I[0];
+    P.I[1] = s.I[1];
+    P.B = s.B;
+    P.resid = b->d_resid + 2 * id;
+    P.n = n_cells;
+    P.m = ds->m;
+    P.steps = pe->plan.steps;
+    P.albedo = albedo;
+    P.one_minus_albedo = 1.0 - albedo;
+    b->slots[id] = s;
+    b->lists_dirty = true;
+    *slot = id;
+    return RSH_OK;
+}
+
+int rsh_batch_remove(rsh_batch b, int32_t slot) {
+    if (!b || slot < 0 || slot >= b->capacity || !b->slots[slot].active) return fail(RSH_E_ARG, "invalid slot");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    Slot& s = b->slots[slot];
+    hipFree(s.I[0]);
+    hipFree(s.I[1]);
+    hipFree(s.B);
+    s = Slot{};
+    BatchProblem& P = b->h_probs[slot];
+    unsigned long long* r = P.resid;
+    P = BatchProblem{};
+    P.resid = r;
+    b->lists_dirty = true;
+    return RSH_OK;
+}
+
+int rsh_batch_run(rsh_batch b, int32_t iterations) {
+    if (!b || iterations < 0) return fail(RSH_E_ARG, "invalid batch_run arguments");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    if (b->lists_dirty) {
+        int rc = rebuild(b);
+        if (rc) return rc;
+    }
+    if (b->classes.empty() || iterations == 0) return RSH_OK;
+    int left = iterations;
+    auto eager = [&]() -> int {
+        int rc = enqueue_iteration(b, b->parity);
+        b->parity ^= 1;
+        left--;
+        return rc;
+    };
+    if (b->parity == 1) {
+        if (int rc = eager()) return rc;
+    }
+    if (b->use_graph && left >= 2) {
+        if (!b->graph_valid) {
+            if (int rc = capture(b)) return rc;
+        }
+        while (left >= 2) {
+            RSH_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            left -= 2;
+        }
+    }
+    while (left > 0) {
+        if (int rc = eager()) return rc;
+    }
+    for (auto& s : b->slots)
+        if (s.active) s.iterations += iterations;
+    return RSH_OK;
+}
+
+int rsh_batch_sync(rsh_batch b) {
+    if (!b) return fail(RSH_E_ARG, "null batch");
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    return RSH_OK;
+}
+
+int rsh_batch_residuals(rsh_batch b, double* resid, int32_t* iterations) {
+    if (!b || !resid || !iterations) return fail(RSH_E_ARG, "null argument");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<unsigned long long> bits(2 * (size_t)b->capacity);
+    RSH_HIP_TRY(hipMemcpy(bits.data(), b->d_resid, bits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    // the last iteration read I[parity ^ 1] and wrote its residual to slot parity ^ 1
+    const int last = b->parity ^ 1;
+    for (int i = 0; i < b->capacity; i++) {
+        const Slot& s = b->slots[i];
+        iterations[i] = s.active ? s.iterations : -1;
+        double v = 0.0;
+        std::memcpy(&v, &bits[2 * (size_t)i + last], sizeof v);
+        resid[i] = (s.active && s.iterations > 0) ? v : -1.0;
+    }
+    return RSH_OK;
+}
+
+int rsh_batch_get_state(rsh_batch b, int32_t slot, double* d_out) {
+    if (!b || slot < 0 || slot >= b->capacity || !b->slots[slot].active || !d_out)
+        return fail(RSH_E_ARG, "invalid slot or output");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    const Slot& s = b->slots[slot];
+    RSH_HIP_TRY(hipMemcpyAsync(d_out, s.I[b->parity], (size_t)s.m * s.n * sizeof(double), hipMemcpyDeviceToDevice,
+                               b->stream));
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    return RSH_OK;
+}
+
+int rsh_batch_stream(rsh_batch b, void** stream) {
+    if (!b || !stream) return fail(RSH_E_ARG, "null argument");
+    *stream = (void*)b->stream;
+    return RSH_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+
+int rsh_autotune_f64(int32_t device, int32_t m, int32_t faces, int64_t n_cells, rsh_tile* best, float* best_ms) {
+    if (m <= 0 || n_cells <= 0 || (faces != 1 && faces != 4) || !best || !best_ms)
+        return fail(RSH_E_ARG, "invalid autotune arguments");
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    std::vector<double> xyz(3 * (size_t)m), w(m);
+    double F = 0;
+    rsh_fibonacci_directions(m, xyz.data(), w.data(), &F);
+    rsh_dirset ds = nullptr;
+    if ((rc = rsh_dirset_create(device, m, xyz.data(), w.data(), F, RSH_PHASE_THOMSON, 0.0, 1, &ds))) return rc;
+    double *illum = nullptr, *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&](int status) {
+        hipFree(illum);
+        hipFree(out);
+        if (st) hipStreamDestroy(st);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        rsh_dirset_destroy(ds);
+        return status;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&illum, (size_t)faces * m * n_cells * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&out, (size_t)m * n_cells * sizeof(double))) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+        return cleanup(hip_fail(e, "autotune allocation"));
+    if ((e = launch_fill_lognormal(illum, (int64_t)faces * m * n_cells, 0.0, 1.0, 0xA070ull, st)) != hipSuccess)
+        return cleanup(hip_fail(e, "autotune fill"));
+    const rsh_tile cands[] = {{8, 2}, {4, 2}, {4, 4}};
+    float bms = -1.f;
+    rsh_tile bt{8, 2};
+    for (const rsh_tile& t : cands) {
+        rc = rsh_scatter_f64_dev(ds, illum, faces, n_cells, 0, m, out, n_cells, st, RSH_F64_MFMA, &t);
+        if (rc) return cleanup(rc);
+        if ((e = hipEventRecord(e0, st)) != hipSuccess) return cleanup(hip_fail(e, "event"));
+        const int reps = 3;
+        for (int r = 0; r < reps; r++) {
+            rc = rsh_scatter_f64_dev(ds, illum, faces, n_cells, 0, m, out, n_cells, st, RSH_F64_MFMA, &t);
+            if (rc) return cleanup(rc);
+        }
+        float ms = 0.f;
+        if ((e = hipEventRecord(e1, st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+            return cleanup(hip_fail(e, "event timing"));
+        ms /= reps;
+        if (bms < 0.f || ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+    }
+    set_tuned_tile(device, plan_f64(m, m).rb_per_pass, bt);
+    *best = bt;
+    *best_ms = bms;
+    return cleanup(RSH_OK);
+}
+
+int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile) {
+    if (!tile || m <= 0) return fail(RSH_E_ARG, "invalid arguments");
+    *tile = f64_tile_for(device, plan_f64(m, m).rb_per_pass);
+    return RSH_OK;
+}
+
+}  // extern "C"

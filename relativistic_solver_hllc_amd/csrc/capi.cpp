@@ -18,7 +18,7 @@
 #include <tuple>
 #include <vector>
 
-#include "rsh_internal.hpp"
+#include "rsh_objects.hpp"
 
 namespace rsh {
 
@@ -43,48 +43,7 @@ using namespace rsh;
 
 // ------------------------------------------------------------------ objects
 
-struct PanelEntry {
-    void* d_panel = nullptr;
-    float* d_row_inv = nullptr;  // split forms: per-row inverse scale
-    F64Plan plan{};
-    SplitPlan splan{};
-    hipEvent_t ready = nullptr;  // recorded after the panel build
-};
-
-struct rsh_dirset_s {
-    int device = 0;
-    int m = 0;
-    double full_area = 0;
-    int phase = 0;
-    double g = 0;
-    int normalize = 0;
-    std::vector<double> xyz, w;
-    double* d_dirs = nullptr;
-    double* d_w = nullptr;
-    double* d_P = nullptr;  // [m][m] FP64 phase matrix
-    std::mutex mu;
-    // key: (k0, k1, faces, form)
-    std::map<std::tuple<int, int, int, int>, PanelEntry> panels;
-    bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
-};
-
-struct rsh_problem_s {
-    int device = 0;
-    int64_t n = 0;
-    int faces = 4;
-    int k_begin = 0, k_end = 0;
-    rsh_dirset ds = nullptr;
-    double* d_illum = nullptr;
-    double* d_S = nullptr;
-    hipStream_t streams[RSH_STREAM_COUNT] = {};
-    hipEvent_t upload_ev[RSH_STREAM_COUNT] = {};
-    bool upload_pending[RSH_STREAM_COUNT] = {};
-    hipEvent_t t0[RSH_STREAM_COUNT] = {}, t1[RSH_STREAM_COUNT] = {};
-    bool timed[RSH_STREAM_COUNT] = {};
-    std::mutex mu;
-};
-
-namespace {
+namespace rsh {
 
 int check_device(int dev) {
     int n = 0;
@@ -146,7 +105,29 @@ int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st
     return RSH_OK;
 }
 
-const rsh_tile kDefaultTile = {8, 2};
+}  // namespace rsh
+
+namespace rsh {
+
+// Tuning table: (device, row blocks per pass) -> FP64 MFMA tile.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, rsh_tile> g_tune;
+
+rsh_tile f64_tile_for(int device, int rb) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find({device, rb});
+    if (it != g_tune.end()) return it->second;
+    return rsh_tile{8, 2};
+}
+
+void set_tuned_tile(int device, int rb, rsh_tile t) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[{device, rb}] = t;
+}
+
+}  // namespace rsh
+
+namespace {
 
 int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int k0, int k1, double* d_out,
                 int64_t ld, hipStream_t st, int kernel, const rsh_tile* tile) {
@@ -163,7 +144,8 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int 
         return RSH_OK;
     }
     if (kernel != RSH_F64_AUTO && kernel != RSH_F64_MFMA) return fail(RSH_E_ARG, "unknown FP64 kernel id");
-    rsh_tile t = (tile && kernel == RSH_F64_MFMA) ? *tile : kDefaultTile;
+    rsh_tile t = (tile && kernel == RSH_F64_MFMA) ? *tile
+                                                  : f64_tile_for(ds->device, plan_f64(ds->m, k1 - k0).rb_per_pass);
     if (!mfma_f64_tile_supported(t.waves, t.chunk_steps))
         return fail(RSH_E_UNSUPPORTED, "unsupported MFMA tile (waves, chunk_steps)");
     const PanelEntry* pe = nullptr;
@@ -379,6 +361,12 @@ int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, cons
                           int32_t out_f32, void* stream) {
     return scatter_split(ds, form, d_state, d_cell_scale, faces, n_cells, k0, k1, d_out, ld_out, out_f32,
                          (hipStream_t)stream);
+}
+
+int rsh_fill_uniform_dev(double* d_out, int64_t count, double lo, double hi, uint64_t seed, void* stream) {
+    if (count < 0 || (count > 0 && !d_out)) return fail(RSH_E_ARG, "invalid fill arguments");
+    RSH_HIP_TRY(launch_fill_uniform(d_out, count, lo, hi, seed, (hipStream_t)stream));
+    return RSH_OK;
 }
 
 int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed, void* stream) {

@@ -19,12 +19,9 @@
 //     fragment), prefetched from L2 one chunk ahead;
 //   - the epilogue storing each 16x16 tile as four 128-byte row segments of the
 //     direction-major output with non-temporal stores (S is not re-read).
-#include "rsh_internal.hpp"
+#include "mfma_f64_core.hpp"
 
 namespace rsh {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
 
 // No FMA contraction: keeps the reference's rounding (3 (1 + mu^2)) / 4 exactly.
 __host__ __device__ inline double phase_kernel(int kind, double g, double mu) {
@@ -185,105 +182,16 @@ template <int RB, int W, int SC, int FACES>
 __global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restrict__ illum, int64_t n, int m,
                                                           const double* __restrict__ panel, int steps,
                                                           double* __restrict__ out, int64_t ld, int rows) {
-    constexpr int FRAG = 64;                       // doubles per fragment
-    constexpr int UNITS = SC * RB * FRAG / 2;      // 16-byte units per LDS stage
-    constexpr int THREADS = 64 * W;
-    constexpr int PER = (UNITS + THREADS - 1) / THREADS;
-    constexpr bool TAIL = (UNITS % THREADS) != 0;
-    __shared__ v2d lds[2][UNITS];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
+    __shared__ v2d lds[2][F64TileShape<RB, W, SC>::UNITS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int jr = lane >> 4;
-    const int cl = lane & 15;
-    const int64_t cell = (int64_t)blockIdx.x * (16 * W) + wave * 16 + cl;
-    const bool cvalid = cell < n;
+    const int64_t cell = (int64_t)blockIdx.x * (16 * W) + wave * 16 + (lane & 15);
     const int rb0 = blockIdx.y * RB;
-
-    // Panel of this pass: steps * RB fragments, stage t = units [t*UNITS, (t+1)*UNITS).
-    const v2d* psrc = reinterpret_cast<const v2d*>(panel + (int64_t)blockIdx.y * steps * RB * FRAG) + tid;
-    // Face pointer of (j = jr, cell); one j-step advances 4 directions.
-    const double* xp = illum + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
-    const int64_t xstep = 4 * n * FACES;
-
     v4d acc[RB];
-#pragma unroll
-    for (int i = 0; i < RB; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
-
-    v2d preg[PER];
-    double xr[SC][FACES];
-    double xb[SC];
-
-#define RSH_LOAD_PANEL(t)                                                     \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
-            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];               \
-    }
-#define RSH_STORE_PANEL(buf)                                                  \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
-            lds[buf][tid + i * THREADS] = preg[i];                            \
-    }
-#define RSH_LOAD_X(t)                                                         \
-    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
-        const int step = (t) * SC + s;                                        \
-        if (cvalid && 4 * step + jr < m) {                                    \
-            const double* p = xp + (int64_t)step * xstep;                     \
-            if constexpr (FACES == 4) {                                       \
-                const v2d a = reinterpret_cast<const v2d*>(p)[0];     \
-                const v2d b = reinterpret_cast<const v2d*>(p)[1];     \
-                xr[s][0] = a.x; xr[s][1 % FACES] = a.y;                       \
-                xr[s][2 % FACES] = b.x; xr[s][3 % FACES] = b.y;               \
-            } else {                                                          \
-                xr[s][0] = p[0];                                              \
-            }                                                                 \
-        } else {                                                              \
-            _Pragma("unroll") for (int f = 0; f < FACES; f++) xr[s][f] = 0.0; \
-        }                                                                     \
-    }
-#define RSH_FACE_SUM()                                                        \
-    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
-        double v = xr[s][0];                                                  \
-        _Pragma("unroll") for (int f = 1; f < FACES; f++) v += xr[s][f];      \
-        xb[s] = v;                                                            \
-    }
-
-    const int nchunks = steps / SC;
-    RSH_LOAD_PANEL(0)
-    RSH_LOAD_X(0)
-    RSH_STORE_PANEL(0)
-    RSH_FACE_SUM()
-    __syncthreads();
-
-    for (int t = 0; t < nchunks; t++) {
-        const int buf = t & 1;
-        const bool more = t + 1 < nchunks;
-        if (more) {
-            RSH_LOAD_PANEL(t + 1)
-            RSH_LOAD_X(t + 1)
-        }
-        const double* a_base = reinterpret_cast<const double*>(lds[buf]) + lane;
-#pragma unroll
-        for (int s = 0; s < SC; s++) {
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const double a = a_base[(s * RB + rb) * FRAG];
-                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[s], acc[rb], 0, 0, 0);
-            }
-        }
-        if (more) {
-            RSH_STORE_PANEL(buf ^ 1)
-            RSH_FACE_SUM()
-        }
-        __syncthreads();
-    }
-#undef RSH_LOAD_PANEL
-#undef RSH_STORE_PANEL
-#undef RSH_LOAD_X
-#undef RSH_FACE_SUM
-
-    if (!cvalid) return;
+    mfma_f64_accumulate<RB, W, SC, FACES>(illum, n, m, panel + (int64_t)blockIdx.y * steps * RB * 64, steps, cell,
+                                          lds, acc);
+    if (cell >= n) return;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
@@ -389,6 +297,23 @@ hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sig
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_lognormal_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d, count, mu, sigma,
                        seed);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void fill_uniform_kernel(double* __restrict__ out, int64_t count, double lo,
+                                                           double hi, uint64_t seed) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t h = splitmix64(seed ^ splitmix64(2 * (uint64_t)i + 1));
+        out[i] = lo + (hi - lo) * ((double)(h >> 11) * 0x1.0p-53);
+    }
+}
+
+hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d, count, lo, hi, seed);
     return hipGetLastError();
 }
 

@@ -1,0 +1,121 @@
+// FP64 MFMA main loop shared by the single-problem kernel and the grouped
+// (batched, persistent) kernels. See kernels_f64.hip for the algorithm.
+#pragma once
+
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+template <int RB, int W, int SC>
+struct F64TileShape {
+    static constexpr int FRAG = 64;                   // doubles per fragment
+    static constexpr int UNITS = SC * RB * FRAG / 2;  // 16-byte units per LDS stage
+};
+
+// Accumulates acc[rb] = sum_j panel(row block rb, j) * X[j][cell] over all
+// steps of one pass for the wave's 16 cells. panel_pass points at the pass's
+// panel (steps * RB fragments). Ends with a workgroup barrier, so the LDS
+// buffers may be reused by the caller right after it returns.
+template <int RB, int W, int SC, int FACES>
+__device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ illum, int64_t n, int m,
+                                                    const double* __restrict__ panel_pass, int steps,
+                                                    int64_t cell, v2d (&lds)[2][F64TileShape<RB, W, SC>::UNITS],
+                                                    v4d (&acc)[RB]) {
+    constexpr int FRAG = 64;
+    constexpr int UNITS = F64TileShape<RB, W, SC>::UNITS;
+    constexpr int THREADS = 64 * W;
+    constexpr int PER = (UNITS + THREADS - 1) / THREADS;
+    constexpr bool TAIL = (UNITS % THREADS) != 0;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int jr = lane >> 4;
+    const bool cvalid = cell < n;
+
+    // Panel of this pass: steps * RB fragments, stage t = units [t*UNITS, (t+1)*UNITS).
+    const v2d* psrc = reinterpret_cast<const v2d*>(panel_pass) + tid;
+    // Face pointer of (j = jr, cell); one j-step advances 4 directions.
+    const double* xp = illum + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
+    const int64_t xstep = 4 * n * FACES;
+
+#pragma unroll
+    for (int i = 0; i < RB; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    v2d preg[PER];
+    double xr[SC][FACES];
+    double xb[SC];
+
+#define RSH_LOAD_PANEL(t)                                                     \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
+            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];               \
+    }
+#define RSH_STORE_PANEL(buf)                                                  \
+    _Pragma("unroll") for (int i = 0; i < PER; i++) {                         \
+        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)               \
+            lds[buf][tid + i * THREADS] = preg[i];                            \
+    }
+#define RSH_LOAD_X(t)                                                         \
+    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
+        const int step = (t) * SC + s;                                        \
+        if (cvalid && 4 * step + jr < m) {                                    \
+            const double* p = xp + (int64_t)step * xstep;                     \
+            if constexpr (FACES == 4) {                                       \
+                const v2d a = reinterpret_cast<const v2d*>(p)[0];     \
+                const v2d b = reinterpret_cast<const v2d*>(p)[1];     \
+                xr[s][0] = a.x; xr[s][1 % FACES] = a.y;                       \
+                xr[s][2 % FACES] = b.x; xr[s][3 % FACES] = b.y;               \
+            } else {                                                          \
+                xr[s][0] = p[0];                                              \
+            }                                                                 \
+        } else {                                                              \
+            _Pragma("unroll") for (int f = 0; f < FACES; f++) xr[s][f] = 0.0; \
+        }                                                                     \
+    }
+#define RSH_FACE_SUM()                                                        \
+    _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
+        double v = xr[s][0];                                                  \
+        _Pragma("unroll") for (int f = 1; f < FACES; f++) v += xr[s][f];      \
+        xb[s] = v;                                                            \
+    }
+
+    const int nchunks = steps / SC;
+    RSH_LOAD_PANEL(0)
+    RSH_LOAD_X(0)
+    RSH_STORE_PANEL(0)
+    RSH_FACE_SUM()
+    __syncthreads();
+
+    for (int t = 0; t < nchunks; t++) {
+        const int buf = t & 1;
+        const bool more = t + 1 < nchunks;
+        if (more) {
+            RSH_LOAD_PANEL(t + 1)
+            RSH_LOAD_X(t + 1)
+        }
+        const double* a_base = reinterpret_cast<const double*>(lds[buf]) + lane;
+#pragma unroll
+        for (int s = 0; s < SC; s++) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const double a = a_base[(s * RB + rb) * FRAG];
+                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[s], acc[rb], 0, 0, 0);
+            }
+        }
+        if (more) {
+            RSH_STORE_PANEL(buf ^ 1)
+            RSH_FACE_SUM()
+        }
+        __syncthreads();
+    }
+#undef RSH_LOAD_PANEL
+#undef RSH_STORE_PANEL
+#undef RSH_LOAD_X
+#undef RSH_FACE_SUM
+
+}
+
+}  // namespace rsh

@@ -78,6 +78,37 @@ hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n
                              const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int rows,
                              void* d_out, int64_t ld, int out_f32, hipStream_t st);
 
+// ------------------------------------------------------------------ batches
+// Device descriptor of one problem of a batch (source iteration on cell-level state).
+struct BatchProblem {
+    const double* panel;        // FP64 phase panel (faces = 1), pass-major
+    double* I[2];               // ping-pong FP64 state [M][N]
+    const double* B;            // source term [M][N]
+    unsigned long long* resid;  // 2 parity slots, bit pattern of a non-negative double
+    int64_t n;
+    int m;
+    int steps;                  // j-steps (FP64) or K-steps (split) of the panel
+    double albedo;
+    double one_minus_albedo;
+    // split forms
+    void* state[2];             // ping-pong packed split state (faces = 1)
+    const float* cell_inv;      // per-cell inverse scale of the state
+    const void* spanel;         // split phase panel
+    const float* row_inv;       // per-row inverse scale of the split panel
+};
+struct BatchItem {
+    int problem;
+    int pass;
+    int64_t tile;  // cell tile of 16 * waves cells
+};
+hipError_t launch_batch_zero(BatchProblem* d_probs, int nprobs, unsigned* d_counters, int ncounters, int parity,
+                             hipStream_t st);
+hipError_t launch_grouped_iter_f64(int rb, int waves, int chunk_steps, int grid, const BatchProblem* d_probs,
+                                   const BatchItem* d_items, const int* d_n_items, unsigned* d_counter, int parity,
+                                   hipStream_t st);
+int grouped_iter_f64_resident(int rb, int waves, int chunk_steps, int device);
+
+hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st);
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
                                  hipStream_t st);
 

@@ -1,0 +1,60 @@
+// Host-side objects behind the C ABI handles (direction sets, problems).
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "rsh_internal.hpp"
+
+struct PanelEntry {
+    void* d_panel = nullptr;
+    float* d_row_inv = nullptr;  // split forms: per-row inverse scale
+    rsh::F64Plan plan{};
+    rsh::SplitPlan splan{};
+    hipEvent_t ready = nullptr;  // recorded after the panel build
+};
+
+struct rsh_dirset_s {
+    int device = 0;
+    int m = 0;
+    double full_area = 0;
+    int phase = 0;
+    double g = 0;
+    int normalize = 0;
+    std::vector<double> xyz, w;
+    double* d_dirs = nullptr;
+    double* d_w = nullptr;
+    double* d_P = nullptr;  // [m][m] FP64 phase matrix
+    std::mutex mu;
+    // key: (k0, k1, faces, form)
+    std::map<std::tuple<int, int, int, int>, PanelEntry> panels;
+    bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
+};
+
+namespace rsh {
+// Phase panel of (k0, k1, faces, form), built on stream st if absent.
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out);
+// Tile of the FP64 MFMA kernel chosen for a row-block class (tuned or default).
+rsh_tile f64_tile_for(int device, int rb);
+void set_tuned_tile(int device, int rb, rsh_tile t);
+int check_device(int dev);
+}  // namespace rsh
+
+struct rsh_problem_s {
+    int device = 0;
+    int64_t n = 0;
+    int faces = 4;
+    int k_begin = 0, k_end = 0;
+    rsh_dirset ds = nullptr;
+    double* d_illum = nullptr;
+    double* d_S = nullptr;
+    hipStream_t streams[RSH_STREAM_COUNT] = {};
+    hipEvent_t upload_ev[RSH_STREAM_COUNT] = {};
+    bool upload_pending[RSH_STREAM_COUNT] = {};
+    hipEvent_t t0[RSH_STREAM_COUNT] = {}, t1[RSH_STREAM_COUNT] = {};
+    bool timed[RSH_STREAM_COUNT] = {};
+    std::mutex mu;
+};
+

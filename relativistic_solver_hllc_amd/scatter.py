@@ -167,6 +167,83 @@ class DirectionSet:
             pass
 
 
+# ------------------------------------------------------------------ autotuner and batches
+
+def autotune_f64(m: int, faces: int = 1, n_cells: int = 65536, device: int = 0) -> Tuple[Tuple[int, int], float]:
+    """Times every FP64 tile for the row-block class of m and records the fastest."""
+    t = N.Tile()
+    ms = C.c_float()
+    N.call("rsh_autotune_f64", device, m, faces, n_cells, C.byref(t), C.byref(ms))
+    return (t.waves, t.chunk_steps), ms.value
+
+
+def tuned_tile_f64(m: int, device: int = 0) -> Tuple[int, int]:
+    t = N.Tile()
+    N.call("rsh_tuned_tile_f64", device, m, C.byref(t))
+    return t.waves, t.chunk_steps
+
+
+class Batch:
+    """Uneven source-iteration problems sharing grouped launches (rsh_batch_*)."""
+
+    def __init__(self, capacity: int, device: int = 0, use_graph: bool = True):
+        h = C.c_void_p()
+        N.call("rsh_batch_create", device, capacity, int(use_graph), C.byref(h))
+        self.handle, self.capacity = h, capacity
+
+    @staticmethod
+    def footprint(form: str, n_cells: int, m: int) -> int:
+        b = C.c_int64()
+        N.call("rsh_batch_footprint", FORMS[form], n_cells, m, C.byref(b))
+        return b.value
+
+    def add(self, ds: DirectionSet, n_cells: int, I0, B, albedo: float, form: str = "fp64") -> int:
+        slot = C.c_int32()
+        N.call("rsh_batch_add", self.handle, ds.handle, n_cells, _ptr(I0), _ptr(B), float(albedo), FORMS[form],
+               C.byref(slot))
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        N.call("rsh_batch_remove", self.handle, slot)
+
+    def run(self, iterations: int) -> None:
+        N.call("rsh_batch_run", self.handle, iterations)
+
+    def sync(self) -> None:
+        N.call("rsh_batch_sync", self.handle)
+
+    def residuals(self) -> Tuple[np.ndarray, np.ndarray]:
+        r = np.empty(self.capacity)
+        it = np.empty(self.capacity, dtype=np.int32)
+        N.call("rsh_batch_residuals", self.handle, r.ctypes.data, it.ctypes.data)
+        return r, it
+
+    def state(self, slot: int, out) -> None:
+        N.call("rsh_batch_get_state", self.handle, slot, _ptr(out))
+
+    def close(self) -> None:
+        if self.handle:
+            N.call("rsh_batch_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fill_uniform_dev(out, lo: float, hi: float, seed: int, stream=None) -> None:
+    N.call("rsh_fill_uniform_dev", _ptr(out), out.numel(), float(lo), float(hi), int(seed) & (2**64 - 1),
+           _stream(stream))
+
+
 # ------------------------------------------------------------------ problem (InitDevice ...)
 
 class Problem:
