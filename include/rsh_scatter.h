@@ -210,7 +210,9 @@ int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
  * and problems join and leave between runs (continuous batching).
  *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM)
  *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot;
- *                        ds holds the directions and phase matrix
+ *                        ds holds the directions and phase matrix; form selects
+ *                        the FP64 state or a packed split state (RSH_FORM_BF16X2,
+ *                        RSH_FORM_FP16X2, RSH_FORM_BF16X3; residual still FP64)
  *   rsh_batch_run        enqueues `iterations` iterations of every active slot
  *                        on the batch's stream (asynchronous)
  *   rsh_batch_residuals  per slot: residual of its last iteration (-1 if none)
@@ -228,6 +230,60 @@ int rsh_batch_sync(rsh_batch b);
 int rsh_batch_residuals(rsh_batch b, double* resid, int32_t* iterations);
 int rsh_batch_get_state(rsh_batch b, int32_t slot, double* d_out);
 int rsh_batch_stream(rsh_batch b, void** stream);
+
+/* ------------------------------------------------ serving loop */
+/* Continuous-batching server of scattering-solve requests. No reference
+ * counterpart (the reference solves one problem per process). A request is a
+ * seeded source-iteration problem: n_cells cells, n_dirs Fibonacci directions,
+ * phase function, albedo; I0 ~ log-normal(0,1), B ~ U(0.5, 2) generated on the
+ * device from the seed; it iterates until its FP64 residual < tolerance (or
+ * max_iterations). Each rsh_server_step admits pending requests in arrival
+ * order while slots and the HBM budget allow, runs iters_per_step iterations of
+ * every resident request in grouped graph launches, and retires finished ones.
+ * rsh_server_run_trace replays arrival times in real time and reports each
+ * request's latency (completion - arrival). */
+typedef enum {
+    RSH_REQ_PENDING = 0,
+    RSH_REQ_RUNNING = 1,
+    RSH_REQ_CONVERGED = 2,
+    RSH_REQ_MAXED = 3,
+    RSH_REQ_CANCELLED = 4
+} rsh_request_status;
+
+typedef struct {
+    int64_t n_cells;
+    int32_t n_dirs;
+    int32_t phase;
+    double g;
+    double albedo;
+    double tolerance;
+    int32_t max_iterations;
+    int32_t form;
+    uint64_t seed;
+} rsh_request_desc;
+
+typedef struct {
+    int32_t status;
+    int32_t iterations;
+    double residual;
+    double t_submit, t_admit, t_done;  /* seconds on the server's monotonic clock */
+    int64_t footprint_bytes;
+} rsh_request_info;
+
+typedef struct rsh_server_s* rsh_server;
+int rsh_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_slots, int32_t iters_per_step,
+                      int32_t keep_results, rsh_server* out);
+int rsh_server_destroy(rsh_server srv);
+int rsh_server_submit(rsh_server srv, const rsh_request_desc* desc, int32_t* request_id);
+int rsh_server_cancel(rsh_server srv, int32_t request_id);
+int rsh_server_step(rsh_server srv, int32_t* running, int32_t* finished_now);
+int rsh_server_poll(rsh_server srv, int32_t request_id, rsh_request_info* info);
+/* Final state [n_dirs][n_cells] FP64 of a finished request (keep_results = 1). */
+int rsh_server_result(rsh_server srv, int32_t request_id, double* d_out);
+int rsh_server_stats(rsh_server srv, int64_t* used_bytes, int32_t* pending, int32_t* running, int64_t* completed);
+double rsh_server_clock(void);
+int rsh_server_run_trace(rsh_server srv, int32_t n, const rsh_request_desc* requests, const double* arrival,
+                         double* latency, double* wall_seconds);
 
 /* -------------------------------------------- reference-shaped problem API */
 

@@ -45,6 +45,32 @@ class ProblemDesc(C.Structure):
     ]
 
 
+class RequestDesc(C.Structure):
+    _fields_ = [
+        ("n_cells", i64),
+        ("n_dirs", i32),
+        ("phase", i32),
+        ("g", f64),
+        ("albedo", f64),
+        ("tolerance", f64),
+        ("max_iterations", i32),
+        ("form", i32),
+        ("seed", u64),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("status", i32),
+        ("iterations", i32),
+        ("residual", f64),
+        ("t_submit", f64),
+        ("t_admit", f64),
+        ("t_done", f64),
+        ("footprint_bytes", i64),
+    ]
+
+
 # name -> (restype, argtypes)
 PROTOTYPES = {
     "rsh_abi_version": (C.c_int, []),
@@ -95,6 +121,16 @@ PROTOTYPES = {
     "rsh_event_destroy": (C.c_int, [vp]),
     "rsh_event_record": (C.c_int, [vp, vp]),
     "rsh_event_elapsed_ms": (C.c_int, [vp, vp, P_f32]),
+    "rsh_server_create": (C.c_int, [i32, i64, i32, i32, i32, P_vp]),
+    "rsh_server_destroy": (C.c_int, [vp]),
+    "rsh_server_submit": (C.c_int, [vp, C.POINTER(RequestDesc), P_i32]),
+    "rsh_server_cancel": (C.c_int, [vp, i32]),
+    "rsh_server_step": (C.c_int, [vp, P_i32, P_i32]),
+    "rsh_server_poll": (C.c_int, [vp, i32, C.POINTER(RequestInfo)]),
+    "rsh_server_result": (C.c_int, [vp, i32, vp]),
+    "rsh_server_stats": (C.c_int, [vp, C.POINTER(i64), P_i32, P_i32, C.POINTER(i64)]),
+    "rsh_server_clock": (f64, []),
+    "rsh_server_run_trace": (C.c_int, [vp, i32, vp, vp, vp, P_f64]),
     "rsh_create": (C.c_int, [C.POINTER(ProblemDesc), P_vp]),
     "rsh_destroy": (C.c_int, [vp]),
     "rsh_upload_illum": (C.c_int, [vp, vp, i64, i64, i64, i32]),

@@ -34,14 +34,18 @@ struct Slot {
     int m = 0;
     int form = RSH_FORM_FP64;
     double albedo = 0;
-    double* I[2] = {nullptr, nullptr};
+    double* I[2] = {nullptr, nullptr};  // FP64 form
+    void* state[2] = {nullptr, nullptr}; // split forms
+    float* cell_inv = nullptr;
     double* B = nullptr;
     F64Plan plan{};
+    SplitPlan splan{};
     int iterations = 0;
     int64_t bytes = 0;
 };
 
 struct ClassState {
+    int form = RSH_FORM_FP64;
     int rb = 0;
     rsh_tile tile{8, 2};
     int grid = 0;
@@ -63,7 +67,7 @@ struct rsh_batch_s {
     BatchProblem* d_probs = nullptr;
     unsigned long long* d_resid = nullptr;  // 2 * capacity
     unsigned* d_counters = nullptr;         // kMaxClasses
-    std::map<int, ClassState> classes;      // by row-block class
+    std::map<std::pair<int, int>, ClassState> classes;  // by (form, row-block class)
     bool lists_dirty = true;
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
@@ -73,6 +77,18 @@ struct rsh_batch_s {
 };
 
 namespace {
+
+void free_slot(Slot& s) {
+    hipFree(s.I[0]);
+    hipFree(s.I[1]);
+    hipFree(s.state[0]);
+    hipFree(s.state[1]);
+    hipFree(s.cell_inv);
+    hipFree(s.B);
+}
+
+int slot_rb(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.rb_per_pass : s.splan.rb_per_pass; }
+int slot_passes(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.passes : s.splan.passes; }
 
 void destroy_graph(rsh_batch b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
@@ -85,7 +101,7 @@ void destroy_graph(rsh_batch b) {
 // Rebuilds the device descriptor table and every class's work list.
 int rebuild(rsh_batch b) {
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
-    std::map<int, std::vector<BatchItem>> lists;
+    std::map<std::pair<int, int>, std::vector<BatchItem>> lists;
     for (auto& kv : b->classes) lists[kv.first];  // keep existing classes (possibly empty)
     // Items of more expensive problems first (longest-processing-time order).
     std::vector<int> order;
@@ -95,22 +111,29 @@ int rebuild(rsh_batch b) {
     bool new_class = false;
     for (int i : order) {
         const Slot& s = b->slots[i];
-        const int rb = s.plan.rb_per_pass;
-        if (!b->classes.count(rb)) {
+        const int rb = slot_rb(s);
+        const auto key = std::make_pair(s.form, rb);
+        if (!b->classes.count(key)) {
             if ((int)b->classes.size() >= kMaxClasses) return fail(RSH_E_UNSUPPORTED, "too many row-block classes");
             ClassState cs;
+            cs.form = s.form;
             cs.rb = rb;
-            cs.tile = f64_tile_for(b->device, rb);
-            cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
+            if (s.form == RSH_FORM_FP64) {
+                cs.tile = f64_tile_for(b->device, rb);
+                cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
+            } else {
+                cs.tile = rsh_tile{grouped_iter_split_waves(), 1};
+                cs.grid = grouped_iter_split_resident(s.form, rb, b->device);
+            }
             cs.counter_index = (int)b->classes.size();
             RSH_HIP_TRY(hipMalloc(&cs.d_n_items, sizeof(int)));
-            b->classes[rb] = cs;
+            b->classes[key] = cs;
             new_class = true;
         }
-        const int tiles_w = 16 * b->classes[rb].tile.waves;
+        const int tiles_w = 16 * b->classes[key].tile.waves;
         const int64_t tiles = (s.n + tiles_w - 1) / tiles_w;
-        auto& L = lists[rb];
-        for (int pass = 0; pass < s.plan.passes; pass++)
+        auto& L = lists[key];
+        for (int pass = 0; pass < slot_passes(s); pass++)
             for (int64_t t = 0; t < tiles; t++) L.push_back(BatchItem{i, pass, t});
     }
     for (auto& kv : b->classes) {
@@ -138,9 +161,13 @@ int enqueue_iteration(rsh_batch b, int parity) {
     RSH_HIP_TRY(launch_batch_zero(b->d_probs, b->capacity, b->d_counters, kMaxClasses, parity, b->stream));
     for (auto& kv : b->classes) {
         ClassState& cs = kv.second;
-        RSH_HIP_TRY(launch_grouped_iter_f64(cs.rb, cs.tile.waves, cs.tile.chunk_steps, cs.grid, b->d_probs,
-                                            cs.d_items, cs.d_n_items, b->d_counters + cs.counter_index, parity,
-                                            b->stream));
+        if (cs.form == RSH_FORM_FP64)
+            RSH_HIP_TRY(launch_grouped_iter_f64(cs.rb, cs.tile.waves, cs.tile.chunk_steps, cs.grid, b->d_probs,
+                                                cs.d_items, cs.d_n_items, b->d_counters + cs.counter_index, parity,
+                                                b->stream));
+        else
+            RSH_HIP_TRY(launch_grouped_iter_split(cs.form, cs.rb, cs.grid, b->d_probs, cs.d_items, cs.d_n_items,
+                                                  b->d_counters + cs.counter_index, parity, b->stream));
     }
     return RSH_OK;
 }
@@ -165,6 +192,8 @@ int capture(rsh_batch b) {
 
 int64_t footprint(int form, int64_t n, int m) {
     if (form == RSH_FORM_FP64) return 3 * (int64_t)m * n * 8;  // I ping-pong + B
+    if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)
+        return 2 * split_state_units(form, 1, n, m) * 16 + (int64_t)m * n * 8 + n * 4;  // state ping-pong + B + scales
     return -1;
 }
 
@@ -213,11 +242,7 @@ int rsh_batch_destroy(rsh_batch b) {
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
     destroy_graph(b);
-    for (auto& s : b->slots) {
-        hipFree(s.I[0]);
-        hipFree(s.I[1]);
-        hipFree(s.B);
-    }
+    for (auto& s : b->slots) free_slot(s);
     for (auto& kv : b->classes) {
         hipFree(kv.second.d_items);
         hipFree(kv.second.d_n_items);
@@ -233,7 +258,7 @@ int rsh_batch_destroy(rsh_batch b) {
 int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B, double albedo,
                   int32_t form, int32_t* slot) {
     if (!b || !ds || !slot || n_cells <= 0 || !d_I0 || !d_B) return fail(RSH_E_ARG, "invalid batch_add arguments");
-    if (form != RSH_FORM_FP64) return fail(RSH_E_UNSUPPORTED, "batches run the FP64 form");
+    if (footprint(form, 1, 1) < 0) return fail(RSH_E_ARG, "unknown form");
     if (ds->device != b->device) return fail(RSH_E_ARG, "direction set lives on another device");
     int id = -1;
     for (int i = 0; i < b->capacity; i++)
@@ -251,35 +276,65 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
     s.form = form;
     s.albedo = albedo;
     s.plan = plan_f64(ds->m, ds->m);
-    const size_t bytes = (size_t)ds->m * n_cells * sizeof(double);
-    hipError_t e;
-    if ((e = hipMalloc(&s.I[0], bytes)) != hipSuccess || (e = hipMalloc(&s.I[1], bytes)) != hipSuccess ||
-        (e = hipMalloc(&s.B, bytes)) != hipSuccess) {
-        hipFree(s.I[0]);
-        hipFree(s.I[1]);
-        hipFree(s.B);
+    s.splan = plan_split(ds->m, ds->m);
+    const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
+    const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
+    hipError_t e = hipMalloc(&s.B, fbytes);
+    if (e == hipSuccess && form == RSH_FORM_FP64) {
+        if ((e = hipMalloc(&s.I[0], fbytes)) == hipSuccess) e = hipMalloc(&s.I[1], fbytes);
+    } else if (e == hipSuccess) {
+        if ((e = hipMalloc(&s.state[0], sbytes)) == hipSuccess && (e = hipMalloc(&s.state[1], sbytes)) == hipSuccess)
+            e = hipMalloc(&s.cell_inv, (size_t)n_cells * sizeof(float));
+    }
+    if (e != hipSuccess) {
+        free_slot(s);
         return hip_fail(e, "batch slot allocation");
     }
-    s.bytes = 3 * (int64_t)bytes;
-    // The next iteration reads I[parity].
-    RSH_HIP_TRY(hipMemcpyAsync(s.I[b->parity], d_I0, bytes, hipMemcpyDeviceToDevice, b->stream));
-    RSH_HIP_TRY(hipMemcpyAsync(s.B, d_B, bytes, hipMemcpyDeviceToDevice, b->stream));
+    s.bytes = footprint(form, n_cells, ds->m);
+    const int p = b->parity;  // the next iteration reads buffer p
+    if ((e = hipMemcpyAsync(s.B, d_B, fbytes, hipMemcpyDeviceToDevice, b->stream)) == hipSuccess) {
+        if (form == RSH_FORM_FP64) {
+            e = hipMemcpyAsync(s.I[p], d_I0, fbytes, hipMemcpyDeviceToDevice, b->stream);
+        } else {
+            // padding positions of the other buffer are never written by the epilogue
+            if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess &&
+                (e = launch_cell_scale(d_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
+                e = launch_pack_state(form, d_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+        }
+    }
+    if (e != hipSuccess) {
+        free_slot(s);
+        return hip_fail(e, "batch slot initialisation");
+    }
     const PanelEntry* pe = nullptr;
-    int rc = get_panel(ds, 0, ds->m, 1, RSH_FORM_FP64, b->stream, &pe);
-    if (rc) return rc;
+    int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe);
+    if (rc) {
+        free_slot(s);
+        return rc;
+    }
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     s.active = true;
     BatchProblem& P = b->h_probs[id];
-    P.panel = static_cast<const double*>(pe->d_panel);
-    P.I[0] = s.I[0];
-    P.I[1] = s.I[1];
+    P = BatchProblem{};
     P.B = s.B;
     P.resid = b->d_resid + 2 * id;
     P.n = n_cells;
     P.m = ds->m;
-    P.steps = pe->plan.steps;
     P.albedo = albedo;
     P.one_minus_albedo = 1.0 - albedo;
+    if (form == RSH_FORM_FP64) {
+        P.panel = static_cast<const double*>(pe->d_panel);
+        P.I[0] = s.I[0];
+        P.I[1] = s.I[1];
+        P.steps = pe->plan.steps;
+    } else {
+        P.spanel = pe->d_panel;
+        P.row_inv = pe->d_row_inv;
+        P.state[0] = s.state[0];
+        P.state[1] = s.state[1];
+        P.cell_inv = s.cell_inv;
+        P.steps = pe->splan.steps;
+    }
     b->slots[id] = s;
     b->lists_dirty = true;
     *slot = id;
@@ -291,9 +346,7 @@ int rsh_batch_remove(rsh_batch b, int32_t slot) {
     RSH_HIP_TRY(hipSetDevice(b->device));
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     Slot& s = b->slots[slot];
-    hipFree(s.I[0]);
-    hipFree(s.I[1]);
-    hipFree(s.B);
+    free_slot(s);
     s = Slot{};
     BatchProblem& P = b->h_probs[slot];
     unsigned long long* r = P.resid;
@@ -367,8 +420,11 @@ int rsh_batch_get_state(rsh_batch b, int32_t slot, double* d_out) {
         return fail(RSH_E_ARG, "invalid slot or output");
     RSH_HIP_TRY(hipSetDevice(b->device));
     const Slot& s = b->slots[slot];
-    RSH_HIP_TRY(hipMemcpyAsync(d_out, s.I[b->parity], (size_t)s.m * s.n * sizeof(double), hipMemcpyDeviceToDevice,
-                               b->stream));
+    if (s.form == RSH_FORM_FP64)
+        RSH_HIP_TRY(hipMemcpyAsync(d_out, s.I[b->parity], (size_t)s.m * s.n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, b->stream));
+    else
+        RSH_HIP_TRY(launch_unpack_state(s.form, s.state[b->parity], s.n, s.m, s.cell_inv, d_out, b->stream));
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     return RSH_OK;
 }

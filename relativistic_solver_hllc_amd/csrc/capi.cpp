@@ -351,7 +351,7 @@ int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64
     if (n_cells == 0) return RSH_OK;
     if (!d_illum || !d_state || !d_cell_scale) return fail(RSH_E_ARG, "null device pointer");
     hipStream_t st = (hipStream_t)stream;
-    RSH_HIP_TRY(launch_cell_scale(d_illum, n_cells, m, faces, d_cell_scale, st));
+    RSH_HIP_TRY(launch_cell_scale(d_illum, nullptr, n_cells, m, faces, d_cell_scale, st));
     RSH_HIP_TRY(launch_pack_state(form, d_illum, faces, n_cells, m, d_cell_scale, d_state, st));
     return RSH_OK;
 }

@@ -28,58 +28,32 @@
 // Phase panel (A operand), per pass of RB row blocks:
 //   panel[(((pass * steps + s) * RB + rb) * NT + t) * 64 + lane] = 16 bytes
 //   lane l, element e <-> row k0 + 16 (pass RB + rb) + l % 16, direction 32 s + 8 (l / 16) + e.
-#include "rsh_internal.hpp"
+#include "mfma_split_core.hpp"
 
 namespace rsh {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef _Float16 v8h __attribute__((ext_vector_type(8)));
-typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-
-constexpr int kTargetExp = 12;  // largest scaled magnitude ~ 2^12
-
-template <int FMT>
-struct Fmt;
-template <>
-struct Fmt<0> {  // BF16
-    typedef __bf16 T;
-    typedef v8bf V;
-    static __device__ inline v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-};
-template <>
-struct Fmt<1> {  // FP16
-    typedef _Float16 T;
-    typedef v8h V;
-    static __device__ inline v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
-};
-
-// Splits x into NT parts of format T with round-to-nearest at each stage;
-// the residual subtraction is exact in FP32.
-template <int FMT, int NT>
-__device__ inline void split_parts(float x, typename Fmt<FMT>::T* parts) {
-    typedef typename Fmt<FMT>::T T;
-    float r = x;
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const T p = (T)r;
-        parts[t] = p;
-        r = r - (float)p;
-    }
-}
-
 // ------------------------------------------------------------------ cell scales
-// inv_scale[c] = 2^-e_c with e_c = kTargetExp - 1 - ilogb(max_{j,f} |I[j][c][f]|).
-__global__ __launch_bounds__(256) void cell_scale_kernel(const double* __restrict__ illum, int64_t n, int m,
+// inv_scale[c] = 2^-e_c with e_c = kTargetExp - 1 - ilogb(max_{j,f} |I[j][c][f]|);
+// an optional second array of the same layout (the source term of an iteration,
+// which bounds every later iterate together with I) joins the maximum.
+__global__ __launch_bounds__(256) void cell_scale_kernel(const double* __restrict__ illum,
+                                                         const double* __restrict__ extra, int64_t n, int m,
                                                          int faces, float* __restrict__ inv_scale) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     double mx = 0.0;
-    const double* p = illum + c * faces;
     const int64_t stride = n * faces;
+    const double* p = illum + c * faces;
     for (int j = 0; j < m; j++) {
         for (int f = 0; f < faces; f++) mx = fmax(mx, fabs(p[f]));
         p += stride;
+    }
+    if (extra) {
+        const double* q = extra + c * faces;
+        for (int j = 0; j < m; j++) {
+            for (int f = 0; f < faces; f++) mx = fmax(mx, fabs(q[f]));
+            q += stride;
+        }
     }
     const int e = mx > 0.0 ? (kTargetExp - 1 - ilogb(mx)) : 0;
     inv_scale[c] = ldexpf(1.0f, -e);
@@ -176,115 +150,17 @@ __global__ __launch_bounds__(64 * W) void split_mfma_kernel(const v4u* __restric
                                                             const v4u* __restrict__ panel,
                                                             const float* __restrict__ row_inv, void* __restrict__ out,
                                                             int64_t ld, int rows, int out_f32) {
-    typedef typename Fmt<FMT>::T T;
-    typedef typename Fmt<FMT>::V V;
-    constexpr int UNITS = RB * NT * 64;  // 16-byte units per LDS stage (one K-step)
-    constexpr int THREADS = 64 * W;
-    constexpr int PER = (UNITS + THREADS - 1) / THREADS;
-    constexpr bool TAIL = (UNITS % THREADS) != 0;
-    constexpr int VX = NT * F;  // state vectors per (cell tile, step)
-    __shared__ v4u lds[2][UNITS];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
+    __shared__ v4u lds[2][SplitTileShape<RB, NT>::UNITS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int64_t ct = (int64_t)blockIdx.x * W + wave;
     const int64_t cell = ct * 16 + (lane & 15);
-    const bool tvalid = ct * 16 < n;
     const int rb0 = blockIdx.y * RB;
-
-    const v4u* psrc = panel + (int64_t)blockIdx.y * steps * UNITS + tid;
-    const v4u* xsrc = state + (tvalid ? ct : 0) * (int64_t)steps * VX * 64 + lane;
-
     v4f accm[RB], accx[RB];
-#pragma unroll
-    for (int i = 0; i < RB; i++) {
-        accm[i] = v4f{0.f, 0.f, 0.f, 0.f};
-        accx[i] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-    v4u preg[PER];
-    v4u xr[VX];
-    V bx[NT];
-
-#define RSH_LOAD_PANEL(t)                                                    \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
-            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];              \
-    }
-#define RSH_STORE_PANEL(buf)                                                 \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
-            lds[buf][tid + i * THREADS] = preg[i];                           \
-    }
-#define RSH_LOAD_X(t)                                                        \
-    if (tvalid) {                                                            \
-        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = xsrc[((int64_t)(t) * VX + v) * 64]; \
-    } else {                                                                 \
-        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = v4u{0u, 0u, 0u, 0u}; \
-    }
-
-    RSH_LOAD_PANEL(0)
-    RSH_LOAD_X(0)
-    RSH_STORE_PANEL(0)
-    __syncthreads();
-
-    for (int t = 0; t < steps; t++) {
-        const int buf = t & 1;
-        const bool more = t + 1 < steps;
-        // B operand of this step from the loaded state vectors
-        if constexpr (F == 1) {
-#pragma unroll
-            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, xr[p]);
-        } else {
-            T parts[NT][8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                float x = 0.0f;
-#pragma unroll
-                for (int f = 0; f < F; f++) {
-                    float fv = 0.0f;
-#pragma unroll
-                    for (int p = NT - 1; p >= 0; p--) fv += (float)__builtin_bit_cast(V, xr[p * F + f])[e];
-                    x += fv;
-                }
-                T q[NT];
-                split_parts<FMT, NT>(x, q);
-#pragma unroll
-                for (int p = 0; p < NT; p++) parts[p][e] = q[p];
-            }
-#pragma unroll
-            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, parts[p]);
-        }
-        if (more) {
-            RSH_LOAD_PANEL(t + 1)
-            RSH_LOAD_X(t + 1)
-        }
-        const v4u* a_base = lds[buf] + lane;
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const V a0 = __builtin_bit_cast(V, a_base[(rb * NT + 0) * 64]);
-            const V a1 = __builtin_bit_cast(V, a_base[(rb * NT + 1) * 64]);
-            accm[rb] = Fmt<FMT>::mfma(a0, bx[0], accm[rb]);
-            if constexpr (NT == 2) {
-                accx[rb] = Fmt<FMT>::mfma(a0, bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a1, bx[0], accx[rb]);
-            } else {
-                const V a2 = __builtin_bit_cast(V, a_base[(rb * NT + 2) * 64]);
-                accx[rb] = Fmt<FMT>::mfma(a1, bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a0, bx[2], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a2, bx[0], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a0, bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a1, bx[0], accx[rb]);
-            }
-        }
-        if (more) RSH_STORE_PANEL(buf ^ 1)
-        __syncthreads();
-    }
-#undef RSH_LOAD_PANEL
-#undef RSH_STORE_PANEL
-#undef RSH_LOAD_X
-
-    if (!tvalid || cell >= n) return;
+    split_accumulate<RB, W, FMT, NT, F>(state, n, steps,
+                                        panel + (int64_t)blockIdx.y * steps * SplitTileShape<RB, NT>::UNITS, ct, lds,
+                                        accm, accx);
+    if (cell >= n) return;
     const double cinv = (double)cell_inv[cell];
     const int g = lane >> 4;
 #pragma unroll
@@ -328,10 +204,11 @@ int64_t split_panel_units(int form, const SplitPlan& pl) {
     return (int64_t)pl.passes * pl.steps * pl.rb_per_pass * split_terms(form) * 64;
 }
 
-hipError_t launch_cell_scale(const double* d_illum, int64_t n, int m, int faces, float* d_inv, hipStream_t st) {
+hipError_t launch_cell_scale(const double* d_illum, const double* d_extra, int64_t n, int m, int faces, float* d_inv,
+                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(cell_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_illum, n, m, faces,
-                       d_inv);
+    hipLaunchKernelGGL(cell_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_illum, d_extra, n, m,
+                       faces, d_inv);
     return hipGetLastError();
 }
 

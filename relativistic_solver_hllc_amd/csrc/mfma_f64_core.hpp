@@ -97,13 +97,21 @@ __device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ i
             RSH_LOAD_X(t + 1)
         }
         const double* a_base = reinterpret_cast<const double*>(lds[buf]) + lane;
+        // A fragments are read PREF fragments ahead of their MFMA so that the
+        // LDS latency hides behind the MFMAs already in the pipe.
+        // Depth: 4 when the accumulators live in AGPRs (4-wave tiles), 2 when
+        // they share the 256 VGPRs of an 8-wave tile.
+        constexpr int NQ = SC * RB;
+        constexpr int PREF_MAX = (W == 4) ? 4 : 2;
+        constexpr int PREF = NQ < PREF_MAX ? NQ : PREF_MAX;
+        double apipe[PREF];
 #pragma unroll
-        for (int s = 0; s < SC; s++) {
+        for (int q = 0; q < PREF; q++) apipe[q] = a_base[q * FRAG];
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const double a = a_base[(s * RB + rb) * FRAG];
-                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[s], acc[rb], 0, 0, 0);
-            }
+        for (int q = 0; q < NQ; q++) {
+            const double a = apipe[q % PREF];
+            if (q + PREF < NQ) apipe[q % PREF] = a_base[(q + PREF) * FRAG];
+            acc[q % RB] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[q / RB], acc[q % RB], 0, 0, 0);
         }
         if (more) {
             RSH_STORE_PANEL(buf ^ 1)

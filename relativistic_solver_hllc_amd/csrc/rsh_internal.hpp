@@ -69,7 +69,8 @@ int split_terms(int form);  // 2 or 3 parts per operand
 int split_fmt(int form);    // 0 = BF16, 1 = FP16
 int64_t split_state_units(int form, int faces, int64_t n, int m);  // 16-byte units
 int64_t split_panel_units(int form, const SplitPlan& pl);
-hipError_t launch_cell_scale(const double* d_illum, int64_t n, int m, int faces, float* d_inv, hipStream_t st);
+hipError_t launch_cell_scale(const double* d_illum, const double* d_extra, int64_t n, int m, int faces,
+                             float* d_inv, hipStream_t st);
 hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t n, int m, const float* d_inv,
                              void* d_state, hipStream_t st);
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
@@ -107,6 +108,13 @@ hipError_t launch_grouped_iter_f64(int rb, int waves, int chunk_steps, int grid,
                                    const BatchItem* d_items, const int* d_n_items, unsigned* d_counter, int parity,
                                    hipStream_t st);
 int grouped_iter_f64_resident(int rb, int waves, int chunk_steps, int device);
+hipError_t launch_grouped_iter_split(int form, int rb, int grid, const BatchProblem* d_probs,
+                                     const BatchItem* d_items, const int* d_n_items, unsigned* d_counter, int parity,
+                                     hipStream_t st);
+int grouped_iter_split_resident(int form, int rb, int device);
+int grouped_iter_split_waves();
+hipError_t launch_unpack_state(int form, const void* d_state, int64_t n, int m, const float* d_cell_inv,
+                               double* d_out, hipStream_t st);
 
 hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st);
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,

@@ -239,6 +239,82 @@ class Batch:
             pass
 
 
+REQ_STATUS = {0: "pending", 1: "running", 2: "converged", 3: "maxed", 4: "cancelled"}
+
+
+def request(n_cells: int, m: int, phase: str = "thomson", g: float = 0.0, albedo: float = 0.5,
+            tolerance: float = 1e-8, max_iterations: int = 200, form: str = "fp64", seed: int = 0) -> N.RequestDesc:
+    return N.RequestDesc(n_cells, m, PHASES[phase], float(g), float(albedo), float(tolerance), max_iterations,
+                         FORMS[form], int(seed) & (2**64 - 1))
+
+
+class Server:
+    """Continuous-batching server of source-iteration requests (rsh_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_slots: int = 64, iters_per_step: int = 2,
+                 keep_results: bool = False, device: int = 0):
+        h = C.c_void_p()
+        N.call("rsh_server_create", device, int(hbm_budget_bytes), max_slots, iters_per_step, int(keep_results),
+               C.byref(h))
+        self.handle = h
+
+    def submit(self, desc: N.RequestDesc) -> int:
+        rid = C.c_int32()
+        N.call("rsh_server_submit", self.handle, C.byref(desc), C.byref(rid))
+        return rid.value
+
+    def cancel(self, rid: int) -> None:
+        N.call("rsh_server_cancel", self.handle, rid)
+
+    def step(self) -> Tuple[int, int]:
+        running, done = C.c_int32(), C.c_int32()
+        N.call("rsh_server_step", self.handle, C.byref(running), C.byref(done))
+        return running.value, done.value
+
+    def poll(self, rid: int) -> dict:
+        info = N.RequestInfo()
+        N.call("rsh_server_poll", self.handle, rid, C.byref(info))
+        return dict(status=REQ_STATUS[info.status], iterations=info.iterations, residual=info.residual,
+                    t_submit=info.t_submit, t_admit=info.t_admit, t_done=info.t_done,
+                    footprint_bytes=info.footprint_bytes)
+
+    def result(self, rid: int, out) -> None:
+        N.call("rsh_server_result", self.handle, rid, _ptr(out))
+
+    def stats(self) -> dict:
+        used, pend, run, comp = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
+        N.call("rsh_server_stats", self.handle, C.byref(used), C.byref(pend), C.byref(run), C.byref(comp))
+        return dict(used_bytes=used.value, pending=pend.value, running=run.value, completed=comp.value)
+
+    def run_trace(self, descs, arrivals) -> Tuple[np.ndarray, float]:
+        """Replays arrival times (seconds, ascending) in real time; returns per-request latency and wall time."""
+        n = len(descs)
+        arr = (N.RequestDesc * n)(*descs)
+        t = np.ascontiguousarray(arrivals, dtype=np.float64)
+        lat = np.empty(n)
+        wall = C.c_double()
+        N.call("rsh_server_run_trace", self.handle, n, C.cast(arr, C.c_void_p), t.ctypes.data, lat.ctypes.data,
+               C.byref(wall))
+        return lat, wall.value
+
+    def close(self) -> None:
+        if self.handle:
+            N.call("rsh_server_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fill_uniform_dev(out, lo: float, hi: float, seed: int, stream=None) -> None:
     N.call("rsh_fill_uniform_dev", _ptr(out), out.numel(), float(lo), float(hi), int(seed) & (2**64 - 1),
            _stream(stream))

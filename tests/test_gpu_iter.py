@@ -125,3 +125,67 @@ def test_autotune_records_a_supported_tile(gpu):
     tile, ms = rs.autotune_f64(400, faces=1, n_cells=32768)
     assert tile in [(8, 2), (4, 2), (4, 4)] and ms > 0
     assert rs.tuned_tile_f64(400) == tile
+
+
+# ---------------------------------------------------------------- split-precision state
+# State error bound after the iterations below, vs the FP64 oracle iteration (relative):
+SPLIT_TOL = {"fp16x2": 2e-6, "bf16x3": 2e-6, "bf16x2": 6e-5}
+
+
+def add_form(batch, p, form, albedo=0.5):
+    dI, dB = D.DeviceArray.from_host(p["I0"]), D.DeviceArray.from_host(p["B"])
+    slot = batch.add(p["ds"], p["n"], dI, dB, albedo, form=form)
+    dI.free()
+    dB.free()
+    return slot
+
+
+@pytest.mark.parametrize("form", ["fp16x2", "bf16x3", "bf16x2"])
+def test_split_iteration_vs_oracle_side_by_side(gpu, form):
+    cases = [(400, 2000, "hg", 0.6), (200, 900, "thomson", 0.0), (800, 333, "hg", 0.9), (161, 257, "hg", 0.3)]
+    probs = [make(m, n, ph, g, seed=40 + i) for i, (m, n, ph, g) in enumerate(cases)]
+    iters = 8
+    with rs.Batch(16) as b:
+        split_slots = [add_form(b, p, form) for p in probs]
+        f64_slots = [add_form(b, p, "fp64") for p in probs]  # same problems in FP64, same launches
+        b.run(iters)
+        r, it = b.residuals()
+        for p, ss, fs in zip(probs, split_slots, f64_slots):
+            ref, rres = orc.source_iteration(p["I0"], p["B"], p["P"], 0.5, iters, threads=16)
+            assert it[ss] == iters and it[fs] == iters
+            err = rel(state(b, ss, p), ref)
+            print(f"{form} M={p['m']} N={p['n']}: state max rel err {err:.2e}")
+            assert err <= SPLIT_TOL[form]
+            assert rel(state(b, fs, p), ref) < 1e-12
+            assert abs(r[ss] - rres[-1]) <= 1e-3 * rres[-1]
+    for p in probs:
+        p["ds"].close()
+
+
+def test_split_alone_equals_in_mix(gpu):
+    probs = [make(m, n, ph, g, seed=60 + i) for i, (m, n, ph, g) in enumerate(MIX)]
+    out = {}
+    for label, members in (("alone", [2]), ("mix", list(range(len(probs))))):
+        with rs.Batch(8) as b:
+            slots = {i: add_form(b, probs[i], "fp16x2") for i in members}
+            b.run(5)
+            out[label] = state(b, slots[2], probs[2])
+    assert np.array_equal(out["alone"], out["mix"])
+    for p in probs:
+        p["ds"].close()
+
+
+@pytest.mark.parametrize("form", ["fp16x2", "bf16x3"])
+def test_split_converges_below_1e_6(gpu, form):
+    p = make(400, 1500, "hg", 0.8, seed=77)
+    with rs.Batch(2) as b:
+        s = add_form(b, p, form)
+        for _ in range(60):
+            b.run(1)
+            r, _ = b.residuals()
+            if r[s] < 1e-6:
+                break
+        assert r[s] < 1e-6
+        ref, _ = orc.source_iteration(p["I0"], p["B"], p["P"], 0.5, int(b.residuals()[1][s]), threads=16)
+        assert rel(state(b, s, p), ref) <= 2e-6
+    p["ds"].close()
