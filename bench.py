@@ -122,6 +122,111 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
     return res
 
 
+def config4_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
+    """Uneven problems: N log-uniform in [n_lo, n_hi] rounded to a structured cube 6 n^3,
+    M in {200, 400, 800}, phase Thomson or HG g in {0.3, 0.6, 0.9}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        edge = max(1, int(round((n_target / 6.0) ** (1.0 / 3.0))))
+        m = int(rng.choice([200, 400, 800]))
+        ph = int(rng.integers(0, 4))
+        phase, g = ("thomson", 0.0) if ph == 0 else ("hg", (0.3, 0.6, 0.9)[ph - 1])
+        out.append(dict(n=6 * edge ** 3, m=m, phase=phase, g=g, seed=int(rng.integers(0, 1 << 30))))
+    return out
+
+
+def batch_section(D, rs, iters=20):
+    """Config 4: 32 uneven problems, `iters` source iterations I <- 0.5 B + 0.5 S each,
+    state resident on the device, grouped graph-replayed launches; FP64 and split forms."""
+    probs = config4_problems()
+    for m in (200, 400, 800):
+        rs.autotune_f64(m, faces=1, n_cells=65536)
+    tiles = {m: rs.tuned_tile_f64(m) for m in (200, 400, 800)}
+    dirsets = {}
+    for p in probs:
+        key = (p["m"], p["phase"], p["g"])
+        if key not in dirsets:
+            xyz, w, F = rs.fibonacci_directions(p["m"])
+            dirsets[key] = rs.DirectionSet(xyz, w, F, phase=p["phase"], g=p["g"], normalize=True)
+    pairs = sum(p["n"] * p["m"] for p in probs)
+    flops = sum(2.0 * p["m"] ** 2 * p["n"] for p in probs)
+    res = {"workload": "config 4: 32 problems, N log-uniform in [1e4, 5e5] as 6 n^3 cells, M in {200,400,800}, "
+                       "Thomson or HG g in {0.3,0.6,0.9} (FP64 row-normalised), I0 ~ log-normal(0,1), "
+                       "B ~ U(0.5,2), albedo 0.5, 20 iterations",
+           "problems": len(probs), "cells_total": sum(p["n"] for p in probs), "pairs_per_iteration": pairs,
+           "iterations": iters, "autotuned_fp64_tiles": {str(k): list(v) for k, v in tiles.items()}, "forms": {}}
+    states = {}
+    for form in ("fp64", "bf16x2", "fp16x2", "bf16x3"):
+        b = rs.Batch(len(probs))
+        slots = []
+        for p in probs:
+            I0, B = D.DeviceArray((p["m"], p["n"])), D.DeviceArray((p["m"], p["n"]))
+            rs.fill_lognormal_dev(I0, 0.0, 1.0, seed=p["seed"] * 2 + 1)
+            rs.fill_uniform_dev(B, 0.5, 2.0, seed=p["seed"] * 2 + 2)
+            slots.append(b.add(dirsets[(p["m"], p["phase"], p["g"])], p["n"], I0, B, 0.5, form=form))
+            I0.free()
+            B.free()
+        b.run(2)  # warm-up: graph capture and first launches
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(iters)
+        b.sync()
+        dt = time.perf_counter() - t0
+        r, it = b.residuals()
+        probe = []
+        for i in (0, 7, 19):
+            out = D.DeviceArray((probs[i]["m"], probs[i]["n"]))
+            b.state(slots[i], out)
+            probe.append(out.to_host())
+            out.free()
+        states[form] = probe
+        entry = {"seconds": dt, "ms_per_iteration": dt * 1e3 / iters, "pairs_per_s": pairs * iters / dt,
+                 "tflops_useful": flops * iters / dt / 1e12, "max_last_residual": float(np.max(r[: len(probs)]))}
+        if form != "fp64":
+            errs = [float(np.max(np.abs(a - f) / np.abs(f))) for a, f in zip(probe, states["fp64"])]
+            entry["max_rel_err_vs_fp64_state"] = max(errs)
+            entry["speedup_vs_fp64"] = res["forms"]["fp64"]["seconds"] / dt
+        res["forms"][form] = entry
+        log(f"[bench] config4 {form}: {dt * 1e3 / iters:.2f} ms/iteration, {pairs * iters / dt:.3e} pairs/s")
+        b.close()
+    for ds in dirsets.values():
+        ds.close()
+    return res
+
+
+def serve_section(rs, rate=10.0, count=64, budget_gb=200):
+    """Config 5: real-time Poisson arrivals of seeded requests drawn as in config 4 with
+    N in [1e4, 1e6]; each iterates to its FP64 residual tolerance; continuous batching."""
+    res = {"workload": f"config 5: Poisson arrivals at {rate:g}/s, {count} requests, N log-uniform in [1e4, 1e6] "
+                       "as 6 n^3, M in {200,400,800}, Thomson or HG; tolerance 1e-8 (fp64) / 1e-6 (split); "
+                       f"HBM budget {budget_gb} GB, 64 slots, 2 iterations per graph launch",
+           "forms": {}}
+    probs = config4_problems(seed=0x5EED + 5, count=count, n_lo=1e4, n_hi=1e6)
+    rng = np.random.default_rng(0x5EED + 55)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    for form, tol in (("fp64", 1e-8), ("bf16x3", 1e-6)):
+        descs = [rs.request(p["n"], p["m"], p["phase"], p["g"], tolerance=tol, form=form, seed=p["seed"])
+                 for p in probs]
+        srv = rs.Server(int(budget_gb * (1 << 30)), 64, iters_per_step=2)
+        lat, wall = srv.run_trace(descs, arrivals)
+        infos = [srv.poll(i) for i in range(count)]
+        srv.close()
+        pairs_it = sum(p["n"] * p["m"] * inf["iterations"] for p, inf in zip(probs, infos))
+        res["forms"][form] = {
+            "tolerance": tol, "wall_s": wall, "requests_per_s": count / wall,
+            "pair_iterations_per_s": pairs_it / wall,
+            "latency_p50_ms": float(np.percentile(lat, 50) * 1e3), "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "latency_max_ms": float(lat.max() * 1e3),
+            "converged": sum(inf["status"] == "converged" for inf in infos),
+            "mean_iterations": float(np.mean([inf["iterations"] for inf in infos])),
+        }
+        log(f"[bench] config5 {form}: p50 {res['forms'][form]['latency_p50_ms']:.1f} ms, "
+            f"p99 {res['forms'][form]['latency_p99_ms']:.1f} ms, {count / wall:.1f} req/s")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +239,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-slice", type=int, default=16)
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision (config 3) section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems (config 4) section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) section")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="before the benchmark, run the plain copy kernel 3x over this many GB "
                          "(reference for FETCH_SIZE / WRITE_SIZE counter scaling)")
@@ -219,6 +326,16 @@ def main() -> int:
         cpu = cpu_baseline(illum, xyz, w, F, n, m, threads, k_slice, "o3")
         log(f"[bench] cpu baseline: {cpu['value']:.3e} pairs/s on {threads} threads")
 
+    batch = None
+    serve = None
+    if not args.no_batch or not args.no_serve:
+        illum.free()
+        out.free()
+    if not args.no_batch:
+        batch = batch_section(D, rs)
+    if not args.no_serve:
+        serve = serve_section(rs)
+
     result = {
         "metric": METRIC,
         "value": pairs / (kern_ms * 1e-3),
@@ -256,6 +373,8 @@ def main() -> int:
         "parity": {"sampled_cells": int(len(cells)), "max_rel_err_vs_oracle": parity, "tolerance": 1e-12},
         "cpu_baseline": cpu,
         "split": split,
+        "batch": batch,
+        "serve": serve,
     }
     ds.close()
     illum.free()

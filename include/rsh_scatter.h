@@ -142,14 +142,16 @@ int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
  *   rsh_pack_split_dev     FP64 illum [M][n_cells][faces] -> state + cell scales
  *   rsh_scatter_split_dev  S rows [k0,k1) from the state, FP32 MFMA accumulation,
  *                          written as FP64 (out_f32 = 0) or FP32 (out_f32 = 1)
- *                          to d_out[(k - k0) * ld_out + c]                       */
+ *                          to d_out[(k - k0) * ld_out + c]; waves = workgroup
+ *                          tile: 4 (one wave per 16-cell tile) or 8 (two waves
+ *                          per tile, each with half of the rows), 0 = default */
 int rsh_split_sizes(int32_t form, int32_t faces, int64_t n_cells, int32_t m, int64_t* state_bytes,
                     int64_t* scale_bytes);
 int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64_t n_cells, int32_t m,
                        void* d_state, float* d_cell_scale, void* stream);
 int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, const float* d_cell_scale,
                           int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
-                          int32_t out_f32, void* stream);
+                          int32_t out_f32, void* stream, int32_t waves);
 
 /* Seeded log-normal fill exp(mu + sigma * N(0,1)) on the device (counter-based
  * generator: the value at index i depends only on (seed, i)). */

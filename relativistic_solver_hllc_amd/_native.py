@@ -87,7 +87,7 @@ PROTOTYPES = {
     "rsh_scatter_f64_dev": (C.c_int, [vp, vp, i32, i64, i32, i32, vp, i64, vp, i32, C.POINTER(Tile)]),
     "rsh_split_sizes": (C.c_int, [i32, i32, i64, i32, C.POINTER(i64), C.POINTER(i64)]),
     "rsh_pack_split_dev": (C.c_int, [i32, vp, i32, i64, i32, vp, vp, vp]),
-    "rsh_scatter_split_dev": (C.c_int, [vp, i32, vp, vp, i32, i64, i32, i32, vp, i64, i32, vp]),
+    "rsh_scatter_split_dev": (C.c_int, [vp, i32, vp, vp, i32, i64, i32, i32, vp, i64, i32, vp, i32]),
     "rsh_fill_lognormal_dev": (C.c_int, [vp, i64, f64, f64, u64, vp]),
     "rsh_fill_uniform_dev": (C.c_int, [vp, i64, f64, f64, u64, vp]),
     "rsh_autotune_f64": (C.c_int, [i32, i32, i32, i64, C.POINTER(Tile), P_f32]),

@@ -156,12 +156,16 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int 
     return RSH_OK;
 }
 
+int split_default_waves() { return 8; }
+
 bool valid_split_form(int form) {
     return form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3;
 }
 
 int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_cell_inv, int faces, int64_t n,
-                  int k0, int k1, void* d_out, int64_t ld, int out_f32, hipStream_t st) {
+                  int k0, int k1, void* d_out, int64_t ld, int out_f32, int waves, hipStream_t st) {
+    if (waves == 0) waves = split_default_waves();
+    if (waves != 4 && waves != 8) return fail(RSH_E_UNSUPPORTED, "split tile: waves must be 4 or 8");
     if (!ds) return fail(RSH_E_ARG, "null direction set");
     if (!valid_split_form(form)) return fail(RSH_E_ARG, "not a split form");
     if (faces != 1 && faces != 4) return fail(RSH_E_UNSUPPORTED, "faces must be 1 or 4");
@@ -174,7 +178,7 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     int rc = get_panel(ds, k0, k1, faces, form, st, &pe);
     if (rc) return rc;
     RSH_HIP_TRY(launch_split_mfma(form, d_state, faces, n, d_cell_inv, pe->d_panel, pe->d_row_inv, pe->splan,
-                                  k1 - k0, d_out, ld, out_f32 ? 1 : 0, st));
+                                  k1 - k0, d_out, ld, out_f32 ? 1 : 0, waves, waves == 8 ? 2 : 1, st));
     return RSH_OK;
 }
 
@@ -358,8 +362,8 @@ int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64
 
 int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, const float* d_cell_scale,
                           int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
-                          int32_t out_f32, void* stream) {
-    return scatter_split(ds, form, d_state, d_cell_scale, faces, n_cells, k0, k1, d_out, ld_out, out_f32,
+                          int32_t out_f32, void* stream, int32_t waves) {
+    return scatter_split(ds, form, d_state, d_cell_scale, faces, n_cells, k0, k1, d_out, ld_out, out_f32, waves,
                          (hipStream_t)stream);
 }
 

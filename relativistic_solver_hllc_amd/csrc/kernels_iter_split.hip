@@ -51,9 +51,9 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
         const int64_t ct = item.tile * W + wave;
         const int64_t cell = ct * 16 + cl;
         v4f accm[RB], accx[RB];
-        split_accumulate<RB, W, FMT, NT, 1>(st_in, P.n, P.steps,
-                                            static_cast<const v4u*>(P.spanel) + (int64_t)item.pass * P.steps * UNITS,
-                                            ct, lds, accm, accx);
+        split_accumulate<RB, W, FMT, NT, 1, 1>(st_in, P.n, P.steps,
+                                               static_cast<const v4u*>(P.spanel) + (int64_t)item.pass * P.steps * UNITS,
+                                               ct, 0, lds, accm, accx);
         double res = 0.0;
         if (cell < P.n) {
             const double cinv = (double)P.cell_inv[cell];

@@ -144,32 +144,38 @@ __global__ __launch_bounds__(256) void split_panel_kernel(const double* __restri
 }
 
 // ------------------------------------------------------------------ contraction
-template <int RB, int W, int FMT, int NT, int F>
+// W waves per workgroup; RH waves share one 16-cell tile (each owns RB / RH of
+// the row blocks), so a workgroup covers 16 W / RH cells.
+template <int RB, int W, int FMT, int NT, int F, int RH>
 __global__ __launch_bounds__(64 * W) void split_mfma_kernel(const v4u* __restrict__ state, int64_t n, int steps,
                                                             const float* __restrict__ cell_inv,
                                                             const v4u* __restrict__ panel,
                                                             const float* __restrict__ row_inv, void* __restrict__ out,
                                                             int64_t ld, int rows, int out_f32) {
+    constexpr int RBW = SplitTileShape<RB, NT, RH>::RBW;
     __shared__ v4u lds[2][SplitTileShape<RB, NT>::UNITS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t ct = (int64_t)blockIdx.x * W + wave;
+    const int h = wave % RH;
+    const int64_t ct = (int64_t)blockIdx.x * (W / RH) + wave / RH;
     const int64_t cell = ct * 16 + (lane & 15);
     const int rb0 = blockIdx.y * RB;
-    v4f accm[RB], accx[RB];
-    split_accumulate<RB, W, FMT, NT, F>(state, n, steps,
-                                        panel + (int64_t)blockIdx.y * steps * SplitTileShape<RB, NT>::UNITS, ct, lds,
-                                        accm, accx);
+    v4f accm[RBW], accx[RBW];
+    split_accumulate<RB, W, FMT, NT, F, RH>(state, n, steps,
+                                            panel + (int64_t)blockIdx.y * steps * SplitTileShape<RB, NT>::UNITS, ct,
+                                            h, lds, accm, accx);
     if (cell >= n) return;
     const double cinv = (double)cell_inv[cell];
     const int g = lane >> 4;
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int i = 0; i < RBW; i++) {
+        const int rb = h * RBW + i;
+        if (rb >= RB) break;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int row = (rb0 + rb) * 16 + 4 * g + r;
             if (row < rows) {
-                const double v = ((double)accm[rb][r] + (double)accx[rb][r]) * (double)row_inv[row] * cinv;
+                const double v = ((double)accm[i][r] + (double)accx[i][r]) * (double)row_inv[row] * cinv;
                 if (out_f32)
                     __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
                 else
@@ -242,14 +248,14 @@ hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1
     return hipGetLastError();
 }
 
-template <int W, int FMT, int NT, int F>
+template <int W, int FMT, int NT, int F, int RH>
 static hipError_t launch_split_rb(int rb, dim3 grid, hipStream_t st, const v4u* state, int64_t n, int steps,
                                   const float* cell_inv, const v4u* panel, const float* row_inv, void* out,
                                   int64_t ld, int rows, int out_f32) {
-#define RSH_RB_CASE(V)                                                                                          \
-    case V:                                                                                                     \
-        hipLaunchKernelGGL((split_mfma_kernel<V, W, FMT, NT, F>), grid, dim3(64 * W), 0, st, state, n, steps,  \
-                           cell_inv, panel, row_inv, out, ld, rows, out_f32);                                   \
+#define RSH_RB_CASE(V)                                                                                             \
+    case V:                                                                                                        \
+        hipLaunchKernelGGL((split_mfma_kernel<V, W, FMT, NT, F, RH>), grid, dim3(64 * W), 0, st, state, n, steps, \
+                           cell_inv, panel, row_inv, out, ld, rows, out_f32);                                      \
         break;
     switch (rb) {
         RSH_RB_CASE(1) RSH_RB_CASE(2) RSH_RB_CASE(3) RSH_RB_CASE(4) RSH_RB_CASE(5) RSH_RB_CASE(6)
@@ -262,21 +268,30 @@ static hipError_t launch_split_rb(int rb, dim3 grid, hipStream_t st, const v4u* 
     return hipGetLastError();
 }
 
+bool split_tile_supported(int waves, int row_split) {
+    return (waves == 4 && row_split == 1) || (waves == 8 && row_split == 2);
+}
+
 hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n, const float* d_cell_inv,
                              const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int rows, void* d_out,
-                             int64_t ld, int out_f32, hipStream_t st) {
+                             int64_t ld, int out_f32, int waves, int row_split, hipStream_t st) {
     if (n <= 0 || rows <= 0) return hipSuccess;
-    constexpr int W = 4;
+    if (!split_tile_supported(waves, row_split)) return hipErrorInvalidValue;
     const int64_t tiles = (n + 15) / 16;
-    const dim3 grid((unsigned)((tiles + W - 1) / W), (unsigned)pl.passes);
+    const int tiles_per_wg = waves / row_split;
+    const dim3 grid((unsigned)((tiles + tiles_per_wg - 1) / tiles_per_wg), (unsigned)pl.passes);
     const v4u* state = static_cast<const v4u*>(d_state);
     const v4u* panel = static_cast<const v4u*>(d_panel);
     const int fmt = split_fmt(form), nt = split_terms(form), rb = pl.rb_per_pass;
-#define RSH_GO(FMT, NT, F) return launch_split_rb<W, FMT, NT, F>(rb, grid, st, state, n, pl.steps, d_cell_inv, panel, d_row_inv, d_out, ld, rows, out_f32)
-    if (fmt == 0 && nt == 2) { if (faces == 4) RSH_GO(0, 2, 4); RSH_GO(0, 2, 1); }
-    if (fmt == 1 && nt == 2) { if (faces == 4) RSH_GO(1, 2, 4); RSH_GO(1, 2, 1); }
-    if (faces == 4) RSH_GO(0, 3, 4);
-    RSH_GO(0, 3, 1);
+#define RSH_GO(W, FMT, NT, F, RH) return launch_split_rb<W, FMT, NT, F, RH>(rb, grid, st, state, n, pl.steps, d_cell_inv, panel, d_row_inv, d_out, ld, rows, out_f32)
+#define RSH_FORMS(W, RH)                                                           \
+    if (fmt == 0 && nt == 2) { if (faces == 4) RSH_GO(W, 0, 2, 4, RH); RSH_GO(W, 0, 2, 1, RH); } \
+    if (fmt == 1 && nt == 2) { if (faces == 4) RSH_GO(W, 1, 2, 4, RH); RSH_GO(W, 1, 2, 1, RH); } \
+    if (faces == 4) RSH_GO(W, 0, 3, 4, RH);                                         \
+    RSH_GO(W, 0, 3, 1, RH);
+    if (waves == 8) { RSH_FORMS(8, 2) }
+    RSH_FORMS(4, 1)
+#undef RSH_FORMS
 #undef RSH_GO
 }
 

@@ -2,6 +2,8 @@
 // grouped source-iteration kernel. See kernels_split.hip for layouts.
 #pragma once
 
+#include <type_traits>
+
 #include "rsh_internal.hpp"
 
 namespace rsh {
@@ -43,73 +45,91 @@ __device__ inline void split_parts(float x, typename Fmt<FMT>::T* parts) {
     }
 }
 
-template <int RB, int NT>
+template <int RB, int NT, int RH = 1>
 struct SplitTileShape {
-    static constexpr int UNITS = RB * NT * 64;  // 16-byte units per LDS stage (one K-step)
+    static constexpr int UNITS = RB * NT * 64;       // 16-byte units per LDS stage (one K-step)
+    static constexpr int RBW = (RB + RH - 1) / RH;   // row blocks per wave
 };
 
-// Accumulates, for the wave's cell tile ct (16 cells) and every row block of one
-// pass, accm[rb] += A_hi B_hi and accx[rb] += cross terms over all K-steps.
-// panel_pass points at the pass's panel (steps * RB * NT fragments). Ends with a
-// workgroup barrier.
-template <int RB, int W, int FMT, int NT, int F>
+// State prefetch depth (K-steps ahead) of a tile: two where the register ring
+// fits without spills, one otherwise.
+template <int RB, int W, int NT, int F, int RH>
+struct SplitPrefetch {
+    static constexpr int RBW = (RB + RH - 1) / RH;
+    static constexpr int XD = (RH == 2 && !(NT == 3 && F == 4)) ? 2 : 1;
+};
+
+// Accumulates, for the wave's cell tile ct (16 cells) and its share of the row
+// blocks of one pass, accm[i] += A_hi B_hi and accx[i] += cross terms over all
+// K-steps. With RH > 1 the RB row blocks of a tile are split over RH waves:
+// wave part h owns row blocks h*RBW .. h*RBW + RBW - 1 (a block past RB re-reads
+// block RB - 1; its result is discarded by the caller). panel_pass points at the
+// pass's panel (steps * RB * NT fragments). Ends with a workgroup barrier.
+template <int RB, int W, int FMT, int NT, int F, int RH = 1>
 __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, int64_t n, int steps,
-                                                 const v4u* __restrict__ panel_pass, int64_t ct,
-                                                 v4u (&lds)[2][SplitTileShape<RB, NT>::UNITS], v4f (&accm)[RB],
-                                                 v4f (&accx)[RB]) {
+                                                 const v4u* __restrict__ panel_pass, int64_t ct, int h,
+                                                 v4u (&lds)[2][SplitTileShape<RB, NT>::UNITS],
+                                                 v4f (&accm)[SplitTileShape<RB, NT, RH>::RBW],
+                                                 v4f (&accx)[SplitTileShape<RB, NT, RH>::RBW]) {
     typedef typename Fmt<FMT>::T T;
     typedef typename Fmt<FMT>::V V;
+    constexpr int RBW = SplitTileShape<RB, NT, RH>::RBW;
     constexpr int UNITS = SplitTileShape<RB, NT>::UNITS;
     constexpr int THREADS = 64 * W;
     constexpr int PER = (UNITS + THREADS - 1) / THREADS;
     constexpr bool TAIL = (UNITS % THREADS) != 0;
     constexpr int VX = NT * F;  // state vectors per (cell tile, step)
+    constexpr int XD = SplitPrefetch<RB, W, NT, F, RH>::XD;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool tvalid = ct * 16 < n;
+    const int rb_first = h * RBW;
 
     const v4u* psrc = panel_pass + tid;
     const v4u* xsrc = state + (tvalid ? ct : 0) * (int64_t)steps * VX * 64 + lane;
 
 #pragma unroll
-    for (int i = 0; i < RB; i++) {
+    for (int i = 0; i < RBW; i++) {
         accm[i] = v4f{0.f, 0.f, 0.f, 0.f};
         accx[i] = v4f{0.f, 0.f, 0.f, 0.f};
     }
     v4u preg[PER];
-    v4u xr[VX];
-    V bx[NT];
+    v4u xr[XD][VX];  // ring of state vectors: slot (t % XD) holds step t
 
-#define RSH_LOAD_PANEL(t)                                                    \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
-            preg[i] = psrc[(int64_t)(t) * UNITS + i * THREADS];              \
-    }
-#define RSH_STORE_PANEL(buf)                                                 \
-    _Pragma("unroll") for (int i = 0; i < PER; i++) {                        \
-        if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS)              \
-            lds[buf][tid + i * THREADS] = preg[i];                           \
-    }
-#define RSH_LOAD_X(t)                                                        \
-    if (tvalid) {                                                            \
-        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = xsrc[((int64_t)(t) * VX + v) * 64]; \
-    } else {                                                                 \
-        _Pragma("unroll") for (int v = 0; v < VX; v++) xr[v] = v4u{0u, 0u, 0u, 0u}; \
-    }
+    auto load_panel = [&](int t) {
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS) preg[i] = psrc[(int64_t)t * UNITS + i * THREADS];
+    };
+    auto store_panel = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (!TAIL || i < PER - 1 || tid + i * THREADS < UNITS) lds[buf][tid + i * THREADS] = preg[i];
+    };
+    auto load_x = [&](auto slot_tag, int t) {
+        constexpr int sl = decltype(slot_tag)::value;
+        if (tvalid) {
+#pragma unroll
+            for (int v = 0; v < VX; v++) xr[sl][v] = xsrc[((int64_t)t * VX + v) * 64];
+        } else {
+#pragma unroll
+            for (int v = 0; v < VX; v++) xr[sl][v] = v4u{0u, 0u, 0u, 0u};
+        }
+    };
 
-    RSH_LOAD_PANEL(0)
-    RSH_LOAD_X(0)
-    RSH_STORE_PANEL(0)
-    __syncthreads();
-
-    for (int t = 0; t < steps; t++) {
+    // One K-step t. SLOT = t % XD is a compile-time constant (the loop below is
+    // unrolled by XD) so the register ring is indexed statically. The panel of
+    // step t + 1 is loaded before the state of step t + XD, so that waiting for
+    // the panel never drains the state prefetch.
+    auto kstep = [&](auto slot_tag, const int t) {
+        constexpr int sl = decltype(slot_tag)::value;
         const int buf = t & 1;
         const bool more = t + 1 < steps;
-        // B operand of this step from the loaded state vectors
+        V bx[NT];
         if constexpr (F == 1) {
 #pragma unroll
-            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, xr[p]);
+            for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, xr[sl][p]);
         } else {
             T parts[NT][8];
 #pragma unroll
@@ -119,7 +139,7 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
                 for (int f = 0; f < F; f++) {
                     float fv = 0.0f;
 #pragma unroll
-                    for (int p = NT - 1; p >= 0; p--) fv += (float)__builtin_bit_cast(V, xr[p * F + f])[e];
+                    for (int p = NT - 1; p >= 0; p--) fv += (float)__builtin_bit_cast(V, xr[sl][p * F + f])[e];
                     x += fv;
                 }
                 T q[NT];
@@ -130,56 +150,67 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
 #pragma unroll
             for (int p = 0; p < NT; p++) bx[p] = __builtin_bit_cast(V, parts[p]);
         }
-        if (more) {
-            RSH_LOAD_PANEL(t + 1)
-            RSH_LOAD_X(t + 1)
-        }
+        if (more) load_panel(t + 1);
+        if (t + XD < steps) load_x(slot_tag, t + XD);
         const v4u* a_base = lds[buf] + lane;
-        // A fragments of row block rb + PREF are read while rb's MFMAs run.
-        constexpr int PREF = RB < 3 ? RB : 3;
+        // A fragments of row block i + PREF are read while i's MFMAs run.
+        constexpr int PREF = RBW < 3 ? RBW : 3;
         V apipe[PREF][NT];
+#define RSH_AFRAG(i, p) __builtin_bit_cast(V, a_base[((rb_first + (i) < RB ? rb_first + (i) : RB - 1) * NT + (p)) * 64])
 #pragma unroll
         for (int q = 0; q < PREF; q++)
 #pragma unroll
-            for (int p = 0; p < NT; p++) apipe[q][p] = __builtin_bit_cast(V, a_base[(q * NT + p) * 64]);
+            for (int p = 0; p < NT; p++) apipe[q][p] = RSH_AFRAG(q, p);
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
+        for (int i = 0; i < RBW; i++) {
             V a[NT];
 #pragma unroll
-            for (int p = 0; p < NT; p++) a[p] = apipe[rb % PREF][p];
-            if (rb + PREF < RB) {
+            for (int p = 0; p < NT; p++) a[p] = apipe[i % PREF][p];
+            if (i + PREF < RBW) {
 #pragma unroll
-                for (int p = 0; p < NT; p++)
-                    apipe[rb % PREF][p] = __builtin_bit_cast(V, a_base[((rb + PREF) * NT + p) * 64]);
+                for (int p = 0; p < NT; p++) apipe[i % PREF][p] = RSH_AFRAG(i + PREF, p);
             }
-            accm[rb] = Fmt<FMT>::mfma(a[0], bx[0], accm[rb]);
+            accm[i] = Fmt<FMT>::mfma(a[0], bx[0], accm[i]);
             if constexpr (NT == 2) {
-                accx[rb] = Fmt<FMT>::mfma(a[0], bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a[1], bx[0], accx[rb]);
+                accx[i] = Fmt<FMT>::mfma(a[0], bx[1], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
             } else {
-                accx[rb] = Fmt<FMT>::mfma(a[1], bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a[0], bx[2], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a[2], bx[0], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a[0], bx[1], accx[rb]);
-                accx[rb] = Fmt<FMT>::mfma(a[1], bx[0], accx[rb]);
+                accx[i] = Fmt<FMT>::mfma(a[1], bx[1], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[0], bx[2], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[2], bx[0], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[0], bx[1], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
             }
         }
+#undef RSH_AFRAG
         // Keep the reads PREF row blocks ahead: the scheduler would otherwise
         // sink each ds_read next to its MFMA. Pattern: PREF*NT reads, then per
-        // row block its MFMAs followed by the reads of row block rb + PREF.
+        // row block its MFMAs followed by the reads of row block i + PREF.
         __builtin_amdgcn_sched_group_barrier(0x100, PREF * NT, 0);
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
+        for (int i = 0; i < RBW; i++) {
             __builtin_amdgcn_sched_group_barrier(0x008, NT == 2 ? 3 : 6, 0);
-            if (rb + PREF < RB) __builtin_amdgcn_sched_group_barrier(0x100, NT, 0);
+            if (i + PREF < RBW) __builtin_amdgcn_sched_group_barrier(0x100, NT, 0);
         }
-        if (more) RSH_STORE_PANEL(buf ^ 1)
+        if (more) store_panel(buf ^ 1);
         __syncthreads();
-    }
-#undef RSH_LOAD_PANEL
-#undef RSH_STORE_PANEL
-#undef RSH_LOAD_X
+    };
 
+    load_panel(0);
+    load_x(std::integral_constant<int, 0>{}, 0);
+    if constexpr (XD == 2) {
+        if (steps > 1) load_x(std::integral_constant<int, 1>{}, 1);
+    }
+    store_panel(0);
+    __syncthreads();
+    if constexpr (XD == 2) {
+        for (int t = 0; t < steps; t += 2) {
+            kstep(std::integral_constant<int, 0>{}, t);
+            if (t + 1 < steps) kstep(std::integral_constant<int, 1>{}, t + 1);
+        }
+    } else {
+        for (int t = 0; t < steps; t++) kstep(std::integral_constant<int, 0>{}, t);
+    }
 }
 
 }  // namespace rsh

@@ -144,10 +144,10 @@ class DirectionSet:
                n_cells if ld is None else ld, _stream(stream), F64_KERNELS[kernel], t)
 
     def scatter_split(self, form: str, state, cell_scale, faces: int, n_cells: int, k0: int, k1: int, out,
-                      ld: Optional[int] = None, out_f32: bool = False, stream=None) -> None:
+                      ld: Optional[int] = None, out_f32: bool = False, stream=None, waves: int = 0) -> None:
         """Split-precision S rows [k0, k1) from a packed state (see pack_split_dev)."""
         N.call("rsh_scatter_split_dev", self.handle, FORMS[form], _ptr(state), _ptr(cell_scale), faces, n_cells, k0,
-               k1, _ptr(out), n_cells if ld is None else ld, int(out_f32), _stream(stream))
+               k1, _ptr(out), n_cells if ld is None else ld, int(out_f32), _stream(stream), waves)
 
     def close(self) -> None:
         if self.handle:

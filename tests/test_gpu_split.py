@@ -94,3 +94,27 @@ def test_split_isotropic_normalised(gpu):
         for form in ("bf16x2", "fp16x2", "bf16x3"):
             got = run_split(ds, form, illum, 4, 0, m)
             assert np.max(np.abs(got - 3.25) / 3.25) <= TOL[form][0]
+
+
+@pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
+def test_split_tiles_bit_identical(gpu, form):
+    """One wave per tile (waves=4) or two waves splitting the rows (waves=8): same sums, same bits."""
+    n, m = 3000, 400
+    rng = np.random.default_rng(31)
+    illum = rng.lognormal(0.0, 1.5, size=(m, n, 4))
+    xyz, w, F = rs.fibonacci_directions(m)
+    with rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True) as ds:
+        d_illum = D.DeviceArray.from_host(illum)
+        sb, cb = rs.split_sizes(form, 4, n, m)
+        state, scale = D.DeviceArray((sb,), np.uint8), D.DeviceArray((cb // 4,), np.float32)
+        rs.pack_split_dev(form, d_illum, 4, n, m, state, scale)
+        outs = []
+        for waves in (4, 8):
+            out = D.DeviceArray((m, n))
+            ds.scatter_split(form, state, scale, 4, n, 0, m, out, waves=waves)
+            D.synchronize()
+            outs.append(out.to_host())
+            out.free()
+        for a in (d_illum, state, scale):
+            a.free()
+    assert np.array_equal(outs[0], outs[1])
