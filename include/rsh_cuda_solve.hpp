@@ -13,16 +13,27 @@
 //   - 64-bit overloads of CudaSendIllumAsync for offsets beyond 2^31;
 //   - the three streams keep the default priority.
 //
+// Radiation moments (CalculateAllParam*, CalculateEnergy, ...) run the
+// library's moments kernel. InitDevice with start == 0 reads the cell geometry
+// from glb_files.base_adress + normals.bin / squares.bin / volume.bin as the
+// reference does (cuda/cuda_struct.cu:137-147) and allocates the pinned host
+// moment arrays grid.divstream / divimpuls / energy / stream / impuls (the
+// reference's ON_FULL_ILLUM_ARRAYS set). Missing geometry files only disable
+// the divergences, which then exit(1) with a message when requested.
+//
 // Types: with RSH_SHIM_REFERENCE_TYPES defined, include the reference's
-// solve_module/solve_global_struct.h before this header and its grid_t,
-// grid_directions_t, e_cuda_stream_id_t and Type are used. Otherwise a minimal
-// layout-compatible set is defined here (same member names; dir[] indexable).
+// solve_module/solve_global_struct.h and global_value.h before this header and
+// its grid_t, grid_directions_t, e_cuda_stream_id_t, Type and glb_files are
+// used. Otherwise a minimal layout-compatible set is defined here (same member
+// names; dir[] indexable) and the program defines `global_files_t glb_files;`
+// (as the reference's main.cpp:6 does).
 #pragma once
 
 #include <cstdint>
 
 #ifndef RSH_SHIM_REFERENCE_TYPES
 #include <array>
+#include <string>
 #include <vector>
 
 typedef double Type;
@@ -40,6 +51,12 @@ struct grid_directions_t {
     std::vector<direction_s> directions;
     Type full_area = 0;
 };
+
+// global_value.h:97-188: only the member the GPU offload reads.
+struct global_files_t {
+    std::string base_adress;
+};
+extern global_files_t glb_files;
 
 void ClearHost(struct grid_t& grid_host);
 

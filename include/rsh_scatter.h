@@ -97,6 +97,17 @@ int rsh_read_directions(const char* path, int32_t cap, int32_t* m, double* xyz, 
 int rsh_write_directions(const char* path, int32_t m, const double* xyz, const double* w,
                          double full_area);
 
+/* Simple-bin file: int32 element count, then the raw elements.
+ * Replaces ReadSimpleFileBin / WriteSimpleFileBin (file_module/reader_bin.h:13-31,
+ * file_module/writer_bin.h:9-27) and, with elem_size = faces * 24 (faces
+ * Vector3 per cell), ReadNormalFile (file_module/reader_bin.cpp:49-69); these
+ * are normals.bin, squares.bin and volume.bin read by InitDevice
+ * (cuda/cuda_struct.cu:137-147). rsh_read_simple_bin stores the file's count
+ * in *count and copies min(count, cap) elements (out may be NULL when cap = 0,
+ * to query the count). */
+int rsh_read_simple_bin(const char* path, int32_t elem_size, void* out, int64_t cap, int64_t* count);
+int rsh_write_simple_bin(const char* path, int32_t elem_size, const void* data, int64_t count);
+
 /* Block partition of n directions over np ranks.
  * Replaces GetSend / GetDisp (solve_module/solve_utils.cpp:177-202). */
 int rsh_partition(int32_t np, int32_t n, int32_t* send, int32_t* disp);
@@ -132,6 +143,28 @@ int rsh_dirset_phase_matrix(rsh_dirset ds, int32_t k0, int32_t k1, double* host_
 int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells,
                         int32_t k0, int32_t k1, double* d_out, int64_t ld_out, void* stream,
                         int32_t kernel, const rsh_tile* tile);
+
+/* Radiation moments over all M directions of ds, per cell, on caller-owned
+ * device memory:
+ *   energy[c]       = sum_j w_j Ibar_j / F
+ *   stream[c][3]    = face average of sum_j w_j omega_j I_jf / F
+ *   impuls[c][9]    = face average of sum_j w_j omega_j omega_j^T I_jf / F
+ *                     (row-major; symmetric)
+ *   divstream[c]    = sum_f (Stream_f . n_f) area_f / volume
+ *   divimpuls[c][3] = sum_f (Impuls_f n_f) area_f / volume
+ * with d_illum [M][n_cells][faces], d_normals [n_cells][faces][3] (outward
+ * normal of each cell face), d_areas [n_cells][faces], d_volume [n_cells].
+ * Any output may be NULL; the geometry may be NULL when both divergences are.
+ * faces_per_thread: 0 = default, else 1, 2 or 4 (faces of one cell handled by
+ * one thread). Requires full_area > 0 in ds.
+ * Replaces d_MakeIllumParam = d_MakeEnergy + d_MakeDivStream + d_MakeDivImpuls
+ * (cuda/cuda_kernel.cu:113-226, helpers cuda/cuda_utils.cu:18-167) with
+ * CUDA_FULL_ARRAYS (cell stream and impuls also written), and d_MakeStream /
+ * d_MakeImpuls (cuda/cuda_kernel.cu:123-147). */
+int rsh_moments_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells,
+                        const double* d_normals, const double* d_areas, const double* d_volume,
+                        double* d_energy, double* d_stream, double* d_impuls, double* d_divstream,
+                        double* d_divimpuls, int32_t faces_per_thread, void* stream);
 
 /* Split-precision forms (RSH_FORM_BF16X2, RSH_FORM_FP16X2, RSH_FORM_BF16X3).
  * No reference counterpart: the reference is FP64 only.
@@ -346,6 +379,31 @@ int rsh_problem_buffers(rsh_problem p, double** d_illum, double** d_scattering);
 
 /* hipStream_t of a stream id. */
 int rsh_problem_stream(rsh_problem p, int32_t stream, void** hip_stream);
+
+/* Cell geometry for the moment divergences, host arrays copied synchronously:
+ * normals [n_cells][faces][3], areas [n_cells][faces], volume [n_cells].
+ * Replaces the start == 0 part of InitDevice (cuda/cuda_struct.cu:63-71,
+ * 85-107, 169-171), which reads them from normals.bin / squares.bin /
+ * volume.bin. */
+int rsh_problem_set_geometry(rsh_problem p, const double* normals, const double* areas,
+                             const double* volume);
+
+/* Radiation moments of the device illum (all M directions) on a stream id,
+ * ordered after every earlier upload. Results stay on the device
+ * (rsh_problem_moment_buffers); each non-NULL host pointer receives its array
+ * by an asynchronous copy on the same stream (energy [N], stream [N][3],
+ * impuls [N][9], divstream [N], divimpuls [N][3]); wait with rsh_sync.
+ * The divergences need rsh_problem_set_geometry first.
+ * Replaces CalculateAllParamStream (cuda/cuda_solve.cu:297-320) and, with a
+ * following rsh_sync, CalculateAllParam / CalculateEnergy / CalculateStream /
+ * CalculateImpuls / CalculateDivStream / CalculateDivImpuls
+ * (cuda/cuda_solve.cu:172-295). */
+int rsh_moments(rsh_problem p, int32_t stream, double* energy, double* stream_out, double* impuls,
+                double* divstream, double* divimpuls);
+
+/* Device arrays of the last rsh_moments (NULL before the first call). */
+int rsh_problem_moment_buffers(rsh_problem p, double** d_energy, double** d_stream, double** d_impuls,
+                               double** d_divstream, double** d_divimpuls);
 
 /* Milliseconds of the last rsh_scatter kernel on each stream id (hipEvents).
  * Replaces CudaEventPrint (cuda/cuda_solve.cu:16-36), whose events the

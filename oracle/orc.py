@@ -31,6 +31,8 @@ _PROTOS = {
     "orc_fibonacci": (None, [i32, vp, vp, C.POINTER(f64)]),
     "orc_read_directions": (C.c_int, [C.c_char_p, i32, C.POINTER(i32), vp, vp, C.POINTER(f64)]),
     "orc_max_threads": (C.c_int, []),
+    "orc_moments": (None, [vp, i64, i32, i32, vp, vp, f64, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
+    "orc_read_simple_bin": (i64, [C.c_char_p, i32, vp, i64]),
 }
 _cache = {}
 
@@ -149,3 +151,36 @@ def gamma(mu):
 
 def phase(kind, g, mu):
     return load().orc_phase(int(kind), float(g), float(mu))
+
+
+MOMENTS = ("energy", "stream", "impuls", "divstream", "divimpuls")
+MOMENT_WIDTH = {"energy": 1, "stream": 3, "impuls": 9, "divstream": 1, "divimpuls": 3}
+
+
+def moments(illum, xyz, w, F, normals=None, areas=None, volume=None, which=MOMENTS, threads=0):
+    """Radiation moments of illum[j][c][f] (orc_moments); returns a dict of the
+    requested outputs, shaped [n] or [n][width]."""
+    illum = _c(illum)
+    m, n, faces = illum.shape
+    geo = [None if a is None else _c(a) for a in (normals, areas, volume)]
+    if any(k.startswith("div") for k in which) and any(a is None for a in geo):
+        raise ValueError("divergences need normals, areas and volume")
+    outs = {k: np.empty((n, MOMENT_WIDTH[k])) for k in which}
+    ptr = lambda a: None if a is None else a.ctypes.data
+    load().orc_moments(illum.ctypes.data, n, m, faces, _c(xyz).ctypes.data, _c(w).ctypes.data, float(F),
+                       ptr(geo[0]), ptr(geo[1]), ptr(geo[2]),
+                       *[ptr(outs.get(k)) for k in MOMENTS], int(threads))
+    return {k: (v[:, 0] if MOMENT_WIDTH[k] == 1 else v) for k, v in outs.items()}
+
+
+def read_simple_bin(path, dtype=np.float64):
+    """Element array of a simple-bin file (int32 count + raw elements)."""
+    dt = np.dtype(dtype)
+    lib = load()
+    n = lib.orc_read_simple_bin(path.encode(), dt.itemsize, None, 0)
+    if n < 0:
+        raise IOError(f"orc_read_simple_bin({path}) failed")
+    out = np.empty(n, dt)
+    if lib.orc_read_simple_bin(path.encode(), dt.itemsize, out.ctypes.data, n) != n:
+        raise IOError(f"orc_read_simple_bin({path}) failed")
+    return out

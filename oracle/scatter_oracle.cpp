@@ -257,4 +257,134 @@ int orc_read_directions(const char* path, int32_t cap, int32_t* m, double* xyz, 
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
+// ---------------------------------------------------------------------------
+// Radiation moments of the face intensities, one cell at a time, in the
+// arithmetic order of the reference's GPU kernels d_MakeIllumParam =
+// d_MakeEnergy + d_MakeDivStream + d_MakeDivImpuls (cuda/cuda_kernel.cu:113-226)
+// with their device helpers (cuda/cuda_utils.cu:18-167), built with
+// CUDA_FULL_ARRAYS so that the cell stream and impuls are also produced:
+//
+//   energy[c]        IntegarteDirection       cuda_utils.cu:18-37
+//                    (face average per direction, * area, summed, / full_area)
+//   Stream_f[c][k]   IntegarteDirection3Faces cuda_utils.cu:132-165
+//                    (I * area, then dir[k] * that, summed, / full_area)
+//   stream[c]        CUDA_CONVERT_FATE_TO_CELL cuda_kernel.cu:9-17 (face average)
+//   divstream[c]     d_MakeDivStream          cuda_kernel.cu:149-178
+//                    (sum_f (sum_k Stream_f[k] n_f[k]) * area_f, / volume)
+//   Impuls_f[c][hk]  IntegarteDirection9Faces cuda_utils.cu:99-130
+//                    (dir[h] * dir[k] * (I * area), summed, / full_area)
+//   impuls[c]        face average, 9 values, index h*3+k (row-major; the
+//                    tensor is symmetric)
+//   divimpuls[c][h]  d_MakeDivImpuls          cuda_kernel.cu:180-217
+//                    (sum_f (sum_k Impuls_f[h*3+k] n_f[k]) * area_f, / volume)
+//
+// Layouts (the reference's device arrays, cuda_struct.cu:63-78,85-107):
+// illum[j][c][f], normals[c][f][3] (outward, per cell face), areas[c][f],
+// volume[c]; energy[c], stream[c][3], impuls[c][9], divstream[c],
+// divimpuls[c][3]. Any output pointer may be null (not computed). The
+// divergences need normals, areas and volume; pass null geometry only with
+// null divergence outputs.
+// ---------------------------------------------------------------------------
+void orc_moments(const double* illum, int64_t n_cells, int32_t n_dirs, int32_t faces, const double* dirs,
+                 const double* w, double full_area, const double* normals, const double* areas,
+                 const double* volume, double* energy, double* stream, double* impuls, double* divstream,
+                 double* divimpuls, int32_t threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n_cells; c++) {
+        if (energy) {
+            double res = 0;
+            for (int32_t i = 0; i < n_dirs; i++) {
+                double I = 0;
+                for (int32_t f = 0; f < faces; f++) I += illum[faces * (n_cells * i + c) + f];
+                I /= faces;
+                res += I * w[i];
+            }
+            energy[c] = res / full_area;
+        }
+        if (stream || divstream) {
+            std::vector<double> S(3 * (size_t)faces, 0.0);
+            for (int32_t i = 0; i < n_dirs; i++) {
+                const double* d = dirs + 3 * i;
+                for (int32_t f = 0; f < faces; f++) {
+                    const double I = illum[faces * (n_cells * i + c) + f];
+                    const double i_a = I * w[i];
+                    S[3 * f + 0] += d[0] * i_a;
+                    S[3 * f + 1] += d[1] * i_a;
+                    S[3 * f + 2] += d[2] * i_a;
+                }
+            }
+            for (int32_t f = 0; f < faces; f++)
+                for (int k = 0; k < 3; k++) S[3 * f + k] /= full_area;
+            if (stream) {
+                for (int k = 0; k < 3; k++) {
+                    double v = 0;
+                    for (int32_t f = 0; f < faces; f++) v += S[3 * f + k];
+                    stream[3 * c + k] = v / faces;
+                }
+            }
+            if (divstream) {
+                double div = 0;
+                for (int32_t f = 0; f < faces; f++) {
+                    double sum = 0;
+                    for (int k = 0; k < 3; k++) sum += S[3 * f + k] * normals[3 * (c * faces + f) + k];
+                    div += sum * areas[c * faces + f];
+                }
+                divstream[c] = div / volume[c];
+            }
+        }
+        if (impuls || divimpuls) {
+            std::vector<double> T(9 * (size_t)faces, 0.0);
+            for (int32_t i = 0; i < n_dirs; i++) {
+                const double* d = dirs + 3 * i;
+                for (int32_t f = 0; f < faces; f++)
+                    for (int h = 0; h < 3; h++)
+                        for (int k = 0; k < 3; k++) {
+                            const double I = illum[faces * (n_cells * i + c) + f];
+                            T[9 * f + 3 * h + k] += d[h] * d[k] * (I * w[i]);
+                        }
+            }
+            for (int32_t f = 0; f < faces; f++)
+                for (int q = 0; q < 9; q++) T[9 * f + q] /= full_area;
+            if (impuls) {
+                for (int q = 0; q < 9; q++) {
+                    double v = 0;
+                    for (int32_t f = 0; f < faces; f++) v += T[9 * f + q];
+                    impuls[9 * c + q] = v / faces;
+                }
+            }
+            if (divimpuls) {
+                double div[3] = {0, 0, 0};
+                for (int32_t f = 0; f < faces; f++)
+                    for (int h = 0; h < 3; h++) {
+                        double sum = 0;
+                        for (int k = 0; k < 3; k++) sum += T[9 * f + 3 * h + k] * normals[3 * (c * faces + f) + k];
+                        div[h] += sum * areas[c * faces + f];
+                    }
+                for (int h = 0; h < 3; h++) divimpuls[3 * c + h] = div[h] / volume[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry binary files read by InitDevice (cuda/cuda_struct.cu:137-147):
+//   simple-bin: int32 count, then count raw elements
+//               (ReadSimpleFileBin, file_module/reader_bin.h:13-31);
+//   normals:    int32 cells, then per cell `faces` Vector3 (3 doubles)
+//               (ReadNormalFile, file_module/reader_bin.cpp:49-69).
+// orc_read_simple_bin returns the element count (or -1 on failure) and copies
+// at most cap elements of elem_size bytes.
+// ---------------------------------------------------------------------------
+int64_t orc_read_simple_bin(const char* path, int32_t elem_size, void* out, int64_t cap) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return -1;
+    int32_t n = 0;
+    if (std::fread(&n, sizeof(int32_t), 1, f) != 1 || n < 0) { std::fclose(f); return -1; }
+    const int64_t take = n < cap ? n : cap;
+    if ((int64_t)std::fread(out, (size_t)elem_size, (size_t)take, f) != take) { std::fclose(f); return -1; }
+    std::fclose(f);
+    return n;
+}
+
 }  // extern "C"

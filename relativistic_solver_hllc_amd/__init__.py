@@ -15,8 +15,10 @@ _EXPORTS = {
     "fill_lognormal_dev": "scatter",
     "partition": "scatter",
     "read_directions": "scatter",
+    "read_simple_bin": "scatter",
     "selftest_mfma_f64": "scatter",
     "write_directions": "scatter",
+    "write_simple_bin": "scatter",
     "DeviceArray": "device",
     "PinnedArray": "device",
     "Stream": "device",

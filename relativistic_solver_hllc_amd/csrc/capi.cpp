@@ -240,6 +240,36 @@ int rsh_write_directions(const char* path, int32_t m, const double* xyz, const d
     return ok ? RSH_OK : fail(RSH_E_IO, "write failed");
 }
 
+int rsh_read_simple_bin(const char* path, int32_t elem_size, void* out, int64_t cap, int64_t* count) {
+    if (!path || !count || elem_size <= 0 || cap < 0 || (cap > 0 && !out)) return fail(RSH_E_ARG, "invalid arguments");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(RSH_E_IO, std::string("cannot open ") + path);
+    int32_t n = 0;
+    int rc = RSH_OK;
+    if (std::fread(&n, sizeof(int32_t), 1, f) != 1 || n < 0) {
+        rc = fail(RSH_E_IO, std::string("bad element count in ") + path);
+    } else {
+        const int64_t take = std::min<int64_t>(n, cap);
+        if (take > 0 && (int64_t)std::fread(out, (size_t)elem_size, (size_t)take, f) != take)
+            rc = fail(RSH_E_IO, std::string("truncated file ") + path);
+    }
+    std::fclose(f);
+    if (rc == RSH_OK) *count = n;
+    return rc;
+}
+
+int rsh_write_simple_bin(const char* path, int32_t elem_size, const void* data, int64_t count) {
+    if (!path || elem_size <= 0 || count < 0 || count > INT32_MAX || (count > 0 && !data))
+        return fail(RSH_E_ARG, "invalid arguments (the format's count is a 32-bit int)");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(RSH_E_IO, std::string("cannot create ") + path);
+    const int32_t n = (int32_t)count;
+    bool ok = std::fwrite(&n, sizeof(int32_t), 1, f) == 1;
+    if (ok && count > 0) ok = (int64_t)std::fwrite(data, (size_t)elem_size, (size_t)count, f) == count;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? RSH_OK : fail(RSH_E_IO, std::string("write failed: ") + path);
+}
+
 int rsh_partition(int32_t np, int32_t n, int32_t* send, int32_t* disp) {
     if (np <= 0 || n < 0 || !send || !disp) return fail(RSH_E_ARG, "invalid partition arguments");
     const int32_t q = n / np, r = n % np;
@@ -294,6 +324,7 @@ int rsh_dirset_create(int32_t device, int32_t m, const double* xyz, const double
         hipFree(ds->d_dirs);
         hipFree(ds->d_w);
         hipFree(ds->d_P);
+        hipFree(ds->d_psi);
         return status;
     };
     hipError_t e;
@@ -308,6 +339,18 @@ int rsh_dirset_create(int32_t device, int32_t m, const double* xyz, const double
             hipSuccess ||
         (e = hipDeviceSynchronize()) != hipSuccess)
         return cleanup(hip_fail(e, "build_phase_kernel"));
+    if (full_area > 0) {
+        // Moment weights psi[j][q] = phi_q(omega_j) * (w_j / F), phi = (1, x, y, z, xx, xy, xz, yy, yz, zz).
+        std::vector<double> psi((size_t)m * kMomentFuncs);
+        for (int j = 0; j < m; j++) {
+            const double a = w[j] / full_area, x = xyz[3 * j], y = xyz[3 * j + 1], z = xyz[3 * j + 2];
+            const double phi[kMomentFuncs] = {1.0, x, y, z, x * x, x * y, x * z, y * y, y * z, z * z};
+            for (int q = 0; q < kMomentFuncs; q++) psi[(size_t)j * kMomentFuncs + q] = phi[q] * a;
+        }
+        if ((e = hipMalloc(&ds->d_psi, psi.size() * sizeof(double))) != hipSuccess ||
+            (e = hipMemcpy(ds->d_psi, psi.data(), psi.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(hip_fail(e, "moment weights"));
+    }
     *out = ds.release();
     return RSH_OK;
 }
@@ -320,6 +363,7 @@ int rsh_dirset_destroy(rsh_dirset ds) {
     hipFree(ds->d_dirs);
     hipFree(ds->d_w);
     hipFree(ds->d_P);
+    hipFree(ds->d_psi);
     delete ds;
     return RSH_OK;
 }
@@ -336,6 +380,32 @@ int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
                         int32_t k1, double* d_out, int64_t ld_out, void* stream, int32_t kernel,
                         const rsh_tile* tile) {
     return scatter_f64(ds, d_illum, faces, n_cells, k0, k1, d_out, ld_out, (hipStream_t)stream, kernel, tile);
+}
+
+static int moments_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, const double* d_normals,
+                       const double* d_areas, const double* d_volume, double* d_energy, double* d_stream,
+                       double* d_impuls, double* d_divstream, double* d_divimpuls, int fpt, hipStream_t st) {
+    if (!ds || (faces != 1 && faces != 4) || n < 0) return fail(RSH_E_ARG, "invalid moment arguments");
+    if (fpt == 0) fpt = 2;
+    if (fpt != 1 && fpt != 2 && fpt != 4) return fail(RSH_E_ARG, "faces_per_thread must be 0, 1, 2 or 4");
+    if (!ds->d_psi) return fail(RSH_E_ARG, "moments need a direction set with full_area > 0");
+    if ((d_divstream || d_divimpuls) && (!d_normals || !d_areas || !d_volume))
+        return fail(RSH_E_ARG, "divergences need normals, areas and volume");
+    if (n == 0) return RSH_OK;
+    if (!d_illum) return fail(RSH_E_ARG, "null device illum");
+    if (reinterpret_cast<uintptr_t>(d_illum) % 16 != 0) fpt = 1;  // 16-byte loads need 16-byte alignment
+    RSH_HIP_TRY(hipSetDevice(ds->device));
+    RSH_HIP_TRY(launch_moments_f64(d_illum, faces, n, ds->m, ds->d_psi, d_normals, d_areas, d_volume, d_energy,
+                                   d_stream, d_impuls, d_divstream, d_divimpuls, fpt, st));
+    return RSH_OK;
+}
+
+int rsh_moments_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells,
+                        const double* d_normals, const double* d_areas, const double* d_volume, double* d_energy,
+                        double* d_stream, double* d_impuls, double* d_divstream, double* d_divimpuls,
+                        int32_t faces_per_thread, void* stream) {
+    return moments_f64(ds, d_illum, faces, n_cells, d_normals, d_areas, d_volume, d_energy, d_stream, d_impuls,
+                       d_divstream, d_divimpuls, faces_per_thread, (hipStream_t)stream);
 }
 
 int rsh_split_sizes(int32_t form, int32_t faces, int64_t n_cells, int32_t m, int64_t* state_bytes,
@@ -613,6 +683,8 @@ int rsh_destroy(rsh_problem p) {
     for (int i = 0; i < RSH_STREAM_COUNT; i++) hipStreamSynchronize(p->streams[i]);
     hipFree(p->d_illum);
     hipFree(p->d_S);
+    hipFree(p->d_geo);
+    hipFree(p->d_mom);
     for (int i = 0; i < RSH_STREAM_COUNT; i++) {
         hipStreamDestroy(p->streams[i]);
         hipEventDestroy(p->upload_ev[i]);
@@ -674,6 +746,64 @@ int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* h
     if (host_out && k1 > k0 && p->n > 0)
         RSH_HIP_TRY(hipMemcpyAsync(host_out + (int64_t)(k0 - p->k_begin) * p->n, dst,
                                    (size_t)(k1 - k0) * p->n * sizeof(double), hipMemcpyDeviceToHost, st));
+    return RSH_OK;
+}
+
+int rsh_problem_set_geometry(rsh_problem p, const double* normals, const double* areas, const double* volume) {
+    if (!p) return fail(RSH_E_ARG, "null problem");
+    const int64_t n = p->n, nf = n * p->faces;
+    if (n > 0 && (!normals || !areas || !volume)) return fail(RSH_E_ARG, "null geometry array");
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    if (!p->d_geo) RSH_HIP_TRY(hipMalloc(&p->d_geo, (size_t)std::max<int64_t>(4 * nf + n, 1) * sizeof(double)));
+    // A moment launch in flight may still read the old geometry.
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) RSH_HIP_TRY(hipStreamSynchronize(p->streams[i]));
+    if (n > 0) {
+        RSH_HIP_TRY(hipMemcpy(p->d_geo, normals, (size_t)3 * nf * sizeof(double), hipMemcpyHostToDevice));
+        RSH_HIP_TRY(hipMemcpy(p->d_geo + 3 * nf, areas, (size_t)nf * sizeof(double), hipMemcpyHostToDevice));
+        RSH_HIP_TRY(hipMemcpy(p->d_geo + 4 * nf, volume, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return RSH_OK;
+}
+
+// Offsets of the moment arrays inside d_mom, in doubles per cell.
+static constexpr int kMomOff[5] = {0, 1, 4, 13, 14};
+static constexpr int kMomWidth[5] = {1, 3, 9, 1, 3};
+static constexpr int kMomPerCell = 17;
+
+int rsh_moments(rsh_problem p, int32_t stream, double* energy, double* stream_out, double* impuls,
+                double* divstream, double* divimpuls) {
+    if (!p || !valid_stream(stream)) return fail(RSH_E_ARG, "invalid problem or stream id");
+    const int64_t n = p->n, nf = n * p->faces;
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    if (!p->d_geo && (divstream || divimpuls))
+        return fail(RSH_E_ARG, "divergences need rsh_problem_set_geometry first");
+    if (!p->d_mom) RSH_HIP_TRY(hipMalloc(&p->d_mom, (size_t)std::max<int64_t>(kMomPerCell * n, 1) * sizeof(double)));
+    hipStream_t st = p->streams[stream];
+    for (int i = 0; i < RSH_STREAM_COUNT; i++)
+        if (p->upload_pending[i] && i != stream) RSH_HIP_TRY(hipStreamWaitEvent(st, p->upload_ev[i], 0));
+    double* dev[5];
+    for (int i = 0; i < 5; i++) dev[i] = p->d_mom + kMomOff[i] * n;
+    const bool geo = p->d_geo != nullptr;
+    int rc = moments_f64(p->ds, p->d_illum, p->faces, n, geo ? p->d_geo : nullptr, geo ? p->d_geo + 3 * nf : nullptr,
+                         geo ? p->d_geo + 4 * nf : nullptr, dev[0], dev[1], dev[2], geo ? dev[3] : nullptr,
+                         geo ? dev[4] : nullptr, 0, st);
+    if (rc) return rc;
+    double* host[5] = {energy, stream_out, impuls, divstream, divimpuls};
+    for (int i = 0; i < 5; i++)
+        if (host[i] && n > 0)
+            RSH_HIP_TRY(hipMemcpyAsync(host[i], dev[i], (size_t)kMomWidth[i] * n * sizeof(double),
+                                       hipMemcpyDeviceToHost, st));
+    return RSH_OK;
+}
+
+int rsh_problem_moment_buffers(rsh_problem p, double** d_energy, double** d_stream, double** d_impuls,
+                               double** d_divstream, double** d_divimpuls) {
+    if (!p) return fail(RSH_E_ARG, "null problem");
+    double** outs[5] = {d_energy, d_stream, d_impuls, d_divstream, d_divimpuls};
+    for (int i = 0; i < 5; i++)
+        if (outs[i]) *outs[i] = p->d_mom ? p->d_mom + kMomOff[i] * p->n : nullptr;
     return RSH_OK;
 }
 

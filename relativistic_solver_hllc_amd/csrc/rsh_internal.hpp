@@ -117,6 +117,14 @@ int grouped_iter_split_waves();
 hipError_t launch_unpack_state(int form, const void* d_state, int64_t n, int m, const float* d_cell_inv,
                                double* d_out, hipStream_t st);
 
+// Radiation moments: psi[j][q] = phi_q(omega_j) w_j / F for
+// phi = (1, x, y, z, xx, xy, xz, yy, yz, zz).
+constexpr int kMomentFuncs = 10;
+hipError_t launch_moments_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_psi,
+                              const double* d_normals, const double* d_areas, const double* d_volume,
+                              double* d_energy, double* d_stream, double* d_impuls, double* d_divstream,
+                              double* d_divimpuls, int faces_per_thread, hipStream_t st);
+
 hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st);
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
                                  hipStream_t st);

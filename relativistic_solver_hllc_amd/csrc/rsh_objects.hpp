@@ -27,6 +27,7 @@ struct rsh_dirset_s {
     double* d_dirs = nullptr;
     double* d_w = nullptr;
     double* d_P = nullptr;  // [m][m] FP64 phase matrix
+    double* d_psi = nullptr;  // [m][kMomentFuncs] moment weights (full_area > 0 only)
     std::mutex mu;
     // key: (k0, k1, faces, form)
     std::map<std::tuple<int, int, int, int>, PanelEntry> panels;
@@ -50,6 +51,8 @@ struct rsh_problem_s {
     rsh_dirset ds = nullptr;
     double* d_illum = nullptr;
     double* d_S = nullptr;
+    double* d_geo = nullptr;  // normals [n][faces][3], areas [n][faces], volume [n]
+    double* d_mom = nullptr;  // energy [n], stream [n][3], impuls [n][9], divstream [n], divimpuls [n][3]
     hipStream_t streams[RSH_STREAM_COUNT] = {};
     hipEvent_t upload_ev[RSH_STREAM_COUNT] = {};
     bool upload_pending[RSH_STREAM_COUNT] = {};

@@ -77,6 +77,26 @@ def write_directions(path: str, xyz, w, full_area: float) -> None:
     N.call("rsh_write_directions", path.encode(), len(w), xyz.ctypes.data, w.ctypes.data, float(full_area))
 
 
+def read_simple_bin(path: str, dtype=np.float64, width: int = 1) -> np.ndarray:
+    """Simple-bin file (int32 count + raw elements of `width` values of dtype):
+    squares.bin / volume.bin (width 1), normals.bin (width = faces * 3)."""
+    dt = np.dtype(dtype)
+    esize = dt.itemsize * width
+    count = C.c_int64()
+    N.call("rsh_read_simple_bin", path.encode(), esize, None, 0, C.byref(count))
+    out = np.empty((count.value, width) if width > 1 else count.value, dt)
+    N.call("rsh_read_simple_bin", path.encode(), esize, out.ctypes.data if out.size else None, count.value,
+           C.byref(count))
+    return out
+
+
+def write_simple_bin(path: str, data, width: int = 1) -> None:
+    """Writes a simple-bin file; the count is len(data) // width elements."""
+    a = np.ascontiguousarray(data)
+    esize = a.dtype.itemsize * width
+    N.call("rsh_write_simple_bin", path.encode(), esize, a.ctypes.data if a.size else None, a.size // width)
+
+
 def partition(np_: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
     send = np.empty(np_, dtype=np.int32)
     disp = np.empty(np_, dtype=np.int32)
@@ -148,6 +168,17 @@ class DirectionSet:
         """Split-precision S rows [k0, k1) from a packed state (see pack_split_dev)."""
         N.call("rsh_scatter_split_dev", self.handle, FORMS[form], _ptr(state), _ptr(cell_scale), faces, n_cells, k0,
                k1, _ptr(out), n_cells if ld is None else ld, int(out_f32), _stream(stream), waves)
+
+    def moments(self, illum, faces: int, n_cells: int, energy=None, stream_out=None, impuls=None, divstream=None,
+                divimpuls=None, normals=None, areas=None, volume=None, faces_per_thread: int = 0,
+                stream=None) -> None:
+        """Radiation moments over all directions on device memory (rsh_moments_f64_dev).
+        Outputs: energy [N], stream [N][3], impuls [N][9], divstream [N], divimpuls [N][3];
+        geometry: normals [N][faces][3], areas [N][faces], volume [N]. None = not computed / absent."""
+        opt = lambda a: None if a is None else _ptr(a)
+        N.call("rsh_moments_f64_dev", self.handle, _ptr(illum), faces, n_cells, opt(normals), opt(areas),
+               opt(volume), opt(energy), opt(stream_out), opt(impuls), opt(divstream), opt(divimpuls),
+               faces_per_thread, _stream(stream))
 
     def close(self) -> None:
         if self.handle:
@@ -359,6 +390,26 @@ class Problem:
         a, b = C.c_void_p(), C.c_void_p()
         N.call("rsh_problem_buffers", self.handle, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_geometry(self, normals, areas, volume) -> None:
+        """normals [N][faces][3], areas [N][faces], volume [N] (InitDevice's normals.bin,
+        squares.bin, volume.bin)."""
+        self._geo = [_f64c(normals), _f64c(areas), _f64c(volume)]
+        N.call("rsh_problem_set_geometry", self.handle, *[a.ctypes.data for a in self._geo])
+
+    def calculate_all_param(self, stream: int = STREAM_PARAMS, energy=None, stream_out=None, impuls=None,
+                            divstream=None, divimpuls=None, sync: bool = True) -> None:
+        """CalculateAllParam(Stream): moments on a stream id, copied into the given host arrays."""
+        opt = lambda a: None if a is None else _ptr(a)
+        N.call("rsh_moments", self.handle, stream, opt(energy), opt(stream_out), opt(impuls), opt(divstream),
+               opt(divimpuls))
+        if sync:
+            N.call("rsh_sync", self.handle, stream)
+
+    def moment_buffers(self) -> Tuple[int, int, int, int, int]:
+        ptrs = [C.c_void_p() for _ in range(5)]
+        N.call("rsh_problem_moment_buffers", self.handle, *[C.byref(p) for p in ptrs])
+        return tuple(p.value for p in ptrs)
 
     def event_times(self) -> np.ndarray:
         ms = (C.c_float * 3)()

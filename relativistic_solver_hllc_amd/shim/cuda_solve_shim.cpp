@@ -4,10 +4,11 @@
 //
 // Build inside the reference tree (replacing cuda/*.cu in makefile:35,163-173):
 //   g++ -std=c++17 -DRSH_SHIM_REFERENCE_TYPES -include solve_module/solve_global_struct.h \
-//       -I<repo>/include -c cuda_solve_shim.cpp && link with -lrsh_scatter
+//       -include global_value.h -I<repo>/include -c cuda_solve_shim.cpp && link with -lrsh_scatter
 // Build stand-alone (tests): g++ -std=c++17 -I<repo>/include cuda_solve_shim.cpp ...
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "rsh_cuda_solve.hpp"
@@ -23,6 +24,7 @@ struct ShimState {
     int k_begin = 0, k_end = 0;
     int64_t n_cells = 0;
     int n_dirs = 0;
+    bool geometry = false;  // normals/squares/volume uploaded (start == 0 and files present)
 };
 ShimState g;
 
@@ -55,6 +57,61 @@ int scatter_local(grid_t& grid, int start, int end, int s) {
         std::exit(1);
     }
     check(rsh_scatter(g.problem, g.k_begin + start, g.k_begin + end, s, grid.scattering), "rsh_scatter");
+    return 0;
+}
+
+// Reads one simple-bin geometry file (int count + raw elements of elem_size
+// bytes) and checks its count; false if it is missing or has the wrong size.
+bool read_geometry_file(const std::string& path, int32_t elem_size, int64_t expect, std::vector<double>& out) {
+    int64_t count = 0;
+    if (rsh_read_simple_bin(path.c_str(), elem_size, nullptr, 0, &count) != RSH_OK) {
+        std::printf("Warning: %s\n", rsh_last_error());
+        return false;
+    }
+    if (count != expect) {
+        std::printf("Warning: %s holds %lld elements, expected %lld\n", path.c_str(), (long long)count,
+                    (long long)expect);
+        return false;
+    }
+    out.resize((size_t)count * (elem_size / sizeof(double)));
+    return rsh_read_simple_bin(path.c_str(), elem_size, out.data(), count, &count) == RSH_OK;
+}
+
+// normals.bin / squares.bin / volume.bin of glb_files.base_adress
+// (cuda/cuda_struct.cu:137-147; ReadNormalFile reader_bin.cpp:49-69 reads
+// `base` Vector3 per cell, ReadSimpleFileBin reader_bin.h:13-31 the others).
+void load_geometry(int64_t n, int faces) {
+    std::vector<double> normals, areas, volume;
+    const std::string base = glb_files.base_adress;
+    const bool ok = read_geometry_file(base + "normals.bin", faces * 3 * (int32_t)sizeof(double), n, normals) &&
+                    read_geometry_file(base + "squares.bin", (int32_t)sizeof(double), n * faces, areas) &&
+                    read_geometry_file(base + "volume.bin", (int32_t)sizeof(double), n, volume);
+    if (!ok) {
+        std::printf("Warning: no cell geometry; divstream/divimpuls unavailable\n");
+        return;
+    }
+    check(rsh_problem_set_geometry(g.problem, normals.data(), areas.data(), volume.data()), "InitDevice(geometry)");
+    g.geometry = true;
+}
+
+// Launches the moments on stream s with the requested host copies (pinned
+// grid arrays); sync waits for them, as the reference's blocking copies do.
+int moments(grid_t& grid, int s, bool energy, bool stream, bool impuls, bool divstream, bool divimpuls, bool sync,
+            const char* where) {
+    require_init(where);
+    if ((divstream || divimpuls) && !g.geometry) {
+        std::printf("Error: %s needs normals.bin/squares.bin/volume.bin (InitDevice with start == 0)\n", where);
+        std::exit(1);
+    }
+    if (!grid.energy || !grid.divstream) {
+        std::printf("Error: %s: moment arrays exist only on the rank with start == 0\n", where);
+        std::exit(1);
+    }
+    check(rsh_moments(g.problem, s, energy ? as_doubles(grid.energy) : nullptr,
+                      stream ? as_doubles(grid.stream) : nullptr, impuls ? as_doubles(grid.impuls) : nullptr,
+                      divstream ? as_doubles(grid.divstream) : nullptr, divimpuls ? as_doubles(grid.divimpuls) : nullptr),
+          where);
+    if (sync) check(rsh_sync(g.problem, s), where);
     return 0;
 }
 
@@ -105,6 +162,20 @@ void InitDevice(const grid_directions_t& grid_dir_host, grid_t& grid_host, const
     check(rsh_host_alloc_pinned((int64_t)(end - start) * grid_host.size * (int64_t)sizeof(double), &p),
           "InitDevice");
     grid_host.scattering = static_cast<Type*>(p);
+    g.geometry = false;
+    if (start == 0) {  // only the first rank computes the moments (cuda_struct.cu:61,158)
+        const int64_t n = grid_host.size;
+        const int widths[5] = {1, 3, 1, 3, 9};  // divstream, divimpuls, energy, stream, impuls
+        void* arr[5] = {};
+        for (int i = 0; i < 5; i++)
+            check(rsh_host_alloc_pinned(widths[i] * n * (int64_t)sizeof(double), &arr[i]), "InitDevice");
+        grid_host.divstream = static_cast<Type*>(arr[0]);
+        grid_host.divimpuls = reinterpret_cast<decltype(grid_host.divimpuls)>(arr[1]);
+        grid_host.energy = static_cast<Type*>(arr[2]);
+        grid_host.stream = reinterpret_cast<decltype(grid_host.stream)>(arr[3]);
+        grid_host.impuls = reinterpret_cast<decltype(grid_host.impuls)>(arr[4]);
+        load_geometry(n, d.faces);
+    }
 }
 
 void ClearDevice() {
@@ -198,19 +269,33 @@ void CudaEventPrint() {
     std::printf("time kernel2 %f\n\n", ms[RSH_STREAM_SCATTERING_2]);
 }
 
-// Radiation moments (cuda/cuda_solve.cu:172-320) are outside the scattering
-// path this library implements; they report an error instead of returning
-// unfilled arrays.
-static int moments_unavailable(const char* name) {
-    std::printf("Error: %s (radiation moments) is not provided by librsh_scatter\n", name);
-    return 1;
+// Radiation moments (cuda/cuda_solve.cu:172-320). The reference's
+// CalculateEnergy / CalculateDivStream / CalculateDivImpuls have their kernel
+// launches commented out and only copy device arrays back; here each computes
+// what its name says. All use stream eCuda_params and return after the copies.
+int CalculateEnergy(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, true, false, false, false, false, true, "CalculateEnergy");
 }
-int CalculateEnergy(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateEnergy"); }
-int CalculateStream(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateStream"); }
-int CalculateImpuls(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateImpuls"); }
-int CalculateDivImpuls(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateDivImpuls"); }
-int CalculateDivStream(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateDivStream"); }
-int CalculateAllParam(const grid_directions_t&, grid_t&) { return moments_unavailable("CalculateAllParam"); }
-int CalculateAllParamStream(const grid_directions_t&, grid_t&, e_cuda_stream_id_t) {
-    return moments_unavailable("CalculateAllParamStream");
+int CalculateStream(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, false, true, false, false, false, true, "CalculateStream");
+}
+int CalculateImpuls(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, false, false, true, false, false, true, "CalculateImpuls");
+}
+// With CUDA_FULL_ARRAYS the reference also copies impuls / stream back (cuda_solve.cu:242-244,265-267).
+int CalculateDivImpuls(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, false, false, true, false, true, true, "CalculateDivImpuls");
+}
+int CalculateDivStream(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, false, true, false, true, false, true, "CalculateDivStream");
+}
+// cuda_solve.cu:272-295: kernel, blocking copies of divstream/divimpuls, and
+// (CUDA_FULL_ARRAYS) energy/stream/impuls.
+int CalculateAllParam(const grid_directions_t&, grid_t& grid) {
+    return moments(grid, eCuda_params, true, true, true, true, true, true, "CalculateAllParam");
+}
+// cuda_solve.cu:297-320: the same on stream st, asynchronously; the caller
+// waits with CudaSyncStream(st) (Illum_async.cpp:692 and later).
+int CalculateAllParamStream(const grid_directions_t&, grid_t& grid, e_cuda_stream_id_t st) {
+    return moments(grid, (int)st, true, true, true, true, true, false, "CalculateAllParamStream");
 }

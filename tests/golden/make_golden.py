@@ -19,6 +19,9 @@ precision arithmetic, independently of the C++ oracle and of the HIP library:
   written as "n // np each, first n % np get one more, displacements = exclusive
   prefix sum" -- a formulation independent of the reference's construction.
 * A direction text file in the format of file_module/reader_txt.cpp:25-47.
+* Radiation moments (energy, flux, pressure tensor and the two divergences;
+  cuda/cuda_kernel.cu:113-226, cuda/cuda_utils.cu:18-167) of random tetrahedra,
+  with fractions.Fraction, rounded once.
 
 Inputs are drawn with numpy's seeded generator and stored in the fixtures, so the
 fixtures are self-contained. Run from the repo root:  python tests/golden/make_golden.py
@@ -120,6 +123,58 @@ def dec_scatter(illum, P):
     return [[sum(P[k][j] * Ibar[j][c] for j in range(M)) for c in range(N)] for k in range(len(P))]
 
 
+def tetra_geometry(rng, n):
+    """Random tetrahedra: outward unit normals [n][4][3] (face f opposite vertex
+    f), face areas [n][4], volumes [n]."""
+    v = rng.uniform(-5, 5, size=(n, 1, 3)) + rng.normal(size=(n, 4, 3))
+    normals, areas = np.empty((n, 4, 3)), np.empty((n, 4))
+    for f, (a, b, c) in enumerate(((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))):
+        cr = np.cross(v[:, b] - v[:, a], v[:, c] - v[:, a])
+        ln = np.linalg.norm(cr, axis=1)
+        nrm = cr / ln[:, None]
+        flip = np.einsum("ij,ij->i", nrm, v[:, f] - (v[:, a] + v[:, b] + v[:, c]) / 3.0) > 0
+        nrm[flip] *= -1.0
+        normals[:, f], areas[:, f] = nrm, 0.5 * ln
+    e1, e2, e3 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]
+    volume = np.abs(np.einsum("ij,ij->i", e1, np.cross(e2, e3))) / 6.0
+    return normals, areas, volume
+
+
+def exact_moments(illum, xyz, w, F, normals, areas, volume):
+    """Energy, face-averaged flux and pressure tensor, and their discrete
+    divergences over the cell faces, in exact rational arithmetic:
+      T_f = sum_j w_j phi(omega_j) I_jf / F for phi = 1, omega, omega omega^T;
+      energy = mean_f T_f[1]; stream = mean_f T_f[omega]; impuls = mean_f T_f[omega omega^T];
+      divstream = sum_f (T_f[omega] . n_f) A_f / V; divimpuls = sum_f T_f[omega omega^T] n_f A_f / V."""
+    M, N, faces = illum.shape
+    X = [[Fraction(float(v)) for v in r] for r in xyz]
+    W = [Fraction(float(v)) for v in w]
+    Ff = Fraction(float(F))
+    out = {k: np.empty(s) for k, s in (("energy", N), ("stream", (N, 3)), ("impuls", (N, 9)),
+                                       ("divstream", N), ("divimpuls", (N, 3)))}
+    for c in range(N):
+        I = [[Fraction(float(illum[j, c, f])) for f in range(faces)] for j in range(M)]
+        e = [sum(W[j] * I[j][f] for j in range(M)) / Ff for f in range(faces)]
+        s = [[sum(W[j] * X[j][k] * I[j][f] for j in range(M)) / Ff for k in range(3)] for f in range(faces)]
+        t = [[[sum(W[j] * X[j][h] * X[j][k] * I[j][f] for j in range(M)) / Ff for k in range(3)]
+              for h in range(3)] for f in range(faces)]
+        nr = [[Fraction(float(normals[c, f, k])) for k in range(3)] for f in range(faces)]
+        A = [Fraction(float(areas[c, f])) for f in range(faces)]
+        V = Fraction(float(volume[c]))
+        out["energy"][c] = float(sum(e) / faces)
+        for k in range(3):
+            out["stream"][c, k] = float(sum(s[f][k] for f in range(faces)) / faces)
+        for h in range(3):
+            for k in range(3):
+                out["impuls"][c, 3 * h + k] = float(sum(t[f][h][k] for f in range(faces)) / faces)
+        out["divstream"][c] = float(sum(sum(s[f][k] * nr[f][k] for k in range(3)) * A[f]
+                                        for f in range(faces)) / V)
+        for h in range(3):
+            out["divimpuls"][c, h] = float(sum(sum(t[f][h][k] * nr[f][k] for k in range(3)) * A[f]
+                                               for f in range(faces)) / V)
+    return out
+
+
 def main():
     rng = np.random.default_rng(0x5EED)
 
@@ -193,6 +248,20 @@ def main():
         for i in range(m):
             f.write(f"{float(w[i])!r} {float(xyz[i, 0])!r} {float(xyz[i, 1])!r} {float(xyz[i, 2])!r}\n")
         f.write(f"{F!r}\n")
+    # 6. Radiation moments (cuda/cuda_kernel.cu:113-226), exact rational
+    # arithmetic, N = 7 random tetrahedra, M = 14 random directions with unequal
+    # weights, F != sum w. Own generator, so the fixtures above do not move.
+    rng6 = np.random.default_rng(0x3017)
+    m6, n6 = 14, 7
+    v6 = rng6.normal(size=(m6, 3))
+    xyz6 = v6 / np.linalg.norm(v6, axis=1, keepdims=True)
+    w6 = rng6.uniform(0.5, 1.5, size=m6)
+    F6 = 13.25
+    illum6 = rng6.lognormal(0.0, 1.0, size=(m6, n6, 4))
+    normals6, areas6, volume6 = tetra_geometry(rng6, n6)
+    mom = exact_moments(illum6, xyz6, w6, F6, normals6, areas6, volume6)
+    np.savez(os.path.join(HERE, "moments_N7_M14.npz"), illum=illum6, xyz=xyz6, w=w6, F=np.array(F6),
+             normals=normals6, areas=areas6, volume=volume6, **mom)
     print("golden fixtures written to", HERE)
 
 

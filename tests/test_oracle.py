@@ -12,7 +12,13 @@ or sign, or a transposed operand fails:
 * invariants: M = 1 closed form, isotropic input, linearity, cell-permutation
   equivariance, joint permutation of input directions with their weights,
   reciprocity w_k P[k][j] = w_j P[j][k] of the symmetric Thomson kernel,
-  GetDisp = exclusive prefix sum of GetSend.
+  GetDisp = exclusive prefix sum of GetSend;
+* radiation moments: exact rational golden vector (energy, flux, pressure
+  tensor, both divergences on random tetrahedra), the isotropic closed form on
+  the six-axis rule (energy = I, flux = 0, pressure = I/3 Id), the divergence
+  theorem for a linear face flux (the reference's TestDivStream check: div = 3),
+  a zero divergence for face-uniform fields (sum_f n_f A_f = 0), face and cell
+  permutation, and the simple-bin geometry file format against numpy.
 
 The reference itself cannot be built here (its CPU path needs Eigen and MPI
 headers absent from the image, its GPU path needs nvcc), so there is no
@@ -25,6 +31,7 @@ import numpy as np
 import pytest
 
 from oracle import orc
+from mesh_util import linear_flux_illum, moment_scales, random_tetra_cells, six_axis_directions
 
 
 def rel(a, b):
@@ -226,3 +233,94 @@ def test_direction_file_golden(golden_dir):
     xyz, w, F = orc.read_directions(os.path.join(golden_dir, "dirs_M26.txt"))
     d = np.load(os.path.join(golden_dir, "tiny_thomson_N64_M26.npz"))
     assert np.array_equal(xyz, d["xyz"]) and np.array_equal(w, d["w"]) and F == float(d["F"])
+
+
+# ---------------------------------------------------------------- radiation moments
+
+def _check_moments(got, ref, illum, w, F, areas, volume, tol):
+    e, d = moment_scales(illum, w, F, areas, volume)
+    for k, v in ref.items():
+        if k not in got:
+            continue
+        scale = d if k.startswith("div") else e
+        err = np.abs(np.asarray(got[k]) - v)
+        err = err if err.ndim == 1 else err.max(axis=1)
+        assert float(np.max(err / scale)) <= tol, k
+
+
+def test_moments_golden_exact(golden_dir):
+    z = np.load(os.path.join(golden_dir, "moments_N7_M14.npz"))
+    got = orc.moments(z["illum"], z["xyz"], z["w"], float(z["F"]), z["normals"], z["areas"], z["volume"])
+    ref = {k: z[k] for k in orc.MOMENTS}
+    _check_moments(got, ref, z["illum"], z["w"], float(z["F"]), z["areas"], z["volume"], 1e-14)
+
+
+def test_moments_isotropic_six_axis():
+    xyz, w, F = six_axis_directions()
+    rng = np.random.default_rng(5)
+    n = 9
+    normals, areas, volume, _ = random_tetra_cells(n, rng)
+    c = rng.uniform(0.5, 3.0, size=n)
+    illum = np.broadcast_to(c[None, :, None], (6, n, 4)).copy()
+    got = orc.moments(illum, xyz, w, F, normals, areas, volume)
+    np.testing.assert_allclose(got["energy"], c, rtol=1e-15)
+    assert np.max(np.abs(got["stream"])) == 0.0
+    np.testing.assert_allclose(got["impuls"], (c[:, None] / 3.0) * np.eye(3).reshape(1, 9), rtol=1e-15, atol=0)
+    # face-uniform fields have zero discrete divergence: sum_f n_f A_f = 0 for a closed cell
+    _, d = moment_scales(illum, w, F, areas, volume)
+    assert np.max(np.abs(got["divstream"]) / d) < 1e-15
+    assert np.max(np.abs(got["divimpuls"]).max(axis=1) / d) < 1e-14
+
+
+def test_moments_divergence_theorem():
+    """Linear face flux F(x) = (3x, 0, 0) at face centroids: discrete divergence 3
+    (solve_module/illum/Illum_async.cpp:1351-1387 sets the same field)."""
+    xyz, w, F = six_axis_directions()
+    rng = np.random.default_rng(11)
+    normals, areas, volume, cents = random_tetra_cells(50, rng)
+    illum = linear_flux_illum(cents, base=2.0, slope=3.0)
+    got = orc.moments(illum, xyz, w, F, normals, areas, volume)
+    _, d = moment_scales(illum, w, F, areas, volume)
+    assert np.max(np.abs(got["divstream"] - 3.0) / d) < 1e-14
+    assert np.max(np.abs(got["divimpuls"] - [3.0, 0.0, 0.0]).max(axis=1) / d) < 1e-14
+    # the face-averaged flux is the field at the mean face centroid
+    np.testing.assert_allclose(got["stream"][:, 0], 3.0 * cents[:, :, 0].mean(axis=1), rtol=1e-13)
+
+
+def test_moments_permutations(golden_dir):
+    z = np.load(os.path.join(golden_dir, "moments_N7_M14.npz"))
+    illum, xyz, w, F = z["illum"], z["xyz"], z["w"], float(z["F"])
+    normals, areas, volume = z["normals"], z["areas"], z["volume"]
+    base = orc.moments(illum, xyz, w, F, normals, areas, volume)
+    rng = np.random.default_rng(3)
+    pc = rng.permutation(illum.shape[1])
+    got = orc.moments(illum[:, pc], xyz, w, F, normals[pc], areas[pc], volume[pc])
+    for k in orc.MOMENTS:
+        np.testing.assert_array_equal(got[k], base[k][pc])
+    pf = rng.permutation(4)
+    got = orc.moments(illum[:, :, pf], xyz, w, F, normals[:, pf], areas[:, pf], volume)
+    _check_moments(got, base, illum, w, F, areas, volume, 1e-14)
+    pj = rng.permutation(illum.shape[0])
+    got = orc.moments(illum[pj], xyz[pj], w[pj], F, normals, areas, volume)
+    _check_moments(got, base, illum, w, F, areas, volume, 1e-14)
+
+
+def test_moments_cell_level_input():
+    """faces = 1: the face average is the value itself; matches faces = 4 with equal faces."""
+    rng = np.random.default_rng(8)
+    xyz, w, F = orc.fibonacci(26)
+    cell = rng.lognormal(size=(26, 5, 1))
+    got1 = orc.moments(cell, xyz, w, F, which=("energy", "stream", "impuls"))
+    got4 = orc.moments(np.repeat(cell, 4, axis=2), xyz, w, F, which=("energy", "stream", "impuls"))
+    for k in got1:
+        np.testing.assert_allclose(got1[k], got4[k], rtol=1e-14, atol=1e-16)
+
+
+def test_simple_bin_format(tmp_path):
+    """int32 count + raw elements (file_module/reader_bin.h:13-31, reader_bin.cpp:49-69)."""
+    rng = np.random.default_rng(2)
+    vol = rng.uniform(size=11)
+    p = str(tmp_path / "volume.bin")
+    with open(p, "wb") as f:
+        f.write(np.int32(11).tobytes() + vol.tobytes())
+    np.testing.assert_array_equal(orc.read_simple_bin(p), vol)

@@ -4,7 +4,9 @@ CPU: every function the header declares is defined by the shim (checked on the
 built driver's symbol table) and the signatures match the reference's.
 GPU: the driver replays the reference's illumination-iteration call sequence
 (Illum_async.cpp:663-1072) through the shim; both the async two-stream result
-and CalculateIntScattering match the oracle.
+and CalculateIntScattering match the oracle, and on the rank with start == 0
+CalculateAllParamStream (with geometry read from normals.bin / squares.bin /
+volume.bin) and CalculateEnergy match the oracle's moments.
 """
 import os
 import re
@@ -13,6 +15,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from mesh_util import moment_scales, random_tetra_cells
 from oracle import orc
 from relativistic_solver_hllc_amd import build as B
 
@@ -60,22 +63,53 @@ def test_shim_defines_every_reference_entry_point():
     assert "CudaSendIllumAsync(long, long, long, double const*)" in defined
 
 
+def write_geometry(dirpath, normals, areas, volume):
+    """normals.bin (int cells + 4 Vector3 per cell), squares.bin, volume.bin
+    (file_module/reader_bin.cpp:49-69, reader_bin.h:13-31)."""
+    n = len(volume)
+    for name, arr, count in (("normals.bin", normals, n), ("squares.bin", areas, areas.size),
+                             ("volume.bin", volume, n)):
+        with open(os.path.join(dirpath, name), "wb") as f:
+            f.write(np.int32(count).tobytes() + np.ascontiguousarray(arr, dtype=np.float64).tobytes())
+
+
 @pytest.mark.gpu
-def test_shim_replays_reference_call_sequence(gpu, tmp_path):
-    n, m, start, end = 2500, 60, 12, 48
+@pytest.mark.parametrize("start,end", [(12, 48), (0, 20)])
+def test_shim_replays_reference_call_sequence(gpu, tmp_path, start, end):
+    n, m = 2500, 60
     rng = np.random.default_rng(21)
     illum = rng.lognormal(0.0, 1.0, size=(m, n, 4))
     xyz, w, F = orc.fibonacci(m)
+    normals, areas, volume, _ = random_tetra_cells(n, rng)
+    write_geometry(tmp_path, normals, areas, volume)
     inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         f.write(np.int64(n).tobytes() + np.int32(m).tobytes() + np.int32(start).tobytes() +
                 np.int32(end).tobytes())
         f.write(xyz.astype(np.float64).tobytes() + w.astype(np.float64).tobytes() + np.float64(F).tobytes())
         f.write(illum.astype(np.float64).tobytes())
-    r = subprocess.run([_driver(), str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([_driver(), str(inp), str(outp), str(tmp_path) + os.sep], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "time kernel1" in r.stdout and "shim_driver ok" in r.stdout
-    res = np.fromfile(outp, dtype=np.float64).reshape(2, end - start, n)
+    raw = np.fromfile(outp, dtype=np.float64)
+    ns = 2 * (end - start) * n
+    res = raw[:ns].reshape(2, end - start, n)
     ref = orc.scatter_reference(illum, xyz, w, F, k_disp=start, k_count=end - start, threads=16)
     for got in res:
         assert float(np.max(np.abs(got - ref) / ref)) < 1e-12
+    if start != 0:
+        assert raw.size == ns  # moments only on the rank with start == 0
+        return
+    assert raw.size == ns + 18 * n
+    mref = orc.moments(illum, xyz, w, F, normals, areas, volume, threads=16)
+    e, d = moment_scales(illum, w, F, areas, volume)
+    off = ns
+    for k, width in (("energy", 1), ("stream", 3), ("impuls", 9), ("divstream", 1), ("divimpuls", 3)):
+        got = raw[off:off + width * n].reshape(n, width) if width > 1 else raw[off:off + n]
+        off += width * n
+        err = np.abs(got - mref[k])
+        err = err if err.ndim == 1 else err.max(axis=1)
+        assert float(np.max(err / (d if k.startswith("div") else e))) < 1e-12, k
+    energy2 = raw[off:]
+    assert np.array_equal(energy2, raw[ns:ns + n])  # CalculateEnergy = energy of CalculateAllParamStream
