@@ -173,15 +173,24 @@ int rsh_moments_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
  * then the contraction reads only that state:
  *   rsh_split_sizes        bytes of the state and of the per-cell scale array
  *   rsh_pack_split_dev     FP64 illum [M][n_cells][faces] -> state + cell scales
+ *   rsh_pack_split_avg_dev FP64 illum [M][n_cells][4] -> cell-level state (faces = 1)
+ *                          of the face average (summed and halved twice in FP64)
  *   rsh_scatter_split_dev  S rows [k0,k1) from the state, FP32 MFMA accumulation,
  *                          written as FP64 (out_f32 = 0) or FP32 (out_f32 = 1)
- *                          to d_out[(k - k0) * ld_out + c]; waves = workgroup
- *                          tile: 4 (one wave per 16-cell tile) or 8 (two waves
- *                          per tile, each with half of the rows), 0 = default */
+ *                          to d_out[(k - k0) * ld_out + c]; waves selects the
+ *                          kernel: 4 or 8 = streaming-panel kernel with 4 waves
+ *                          per workgroup (one per 16-cell tile) or 8 (two per
+ *                          tile, each with half of the rows); 1 = resident-panel
+ *                          kernel (faces = 1 only: each persistent workgroup
+ *                          keeps its rows' panel for all directions in LDS);
+ *                          0 = resident-panel kernel where it applies, else 8.
+ *                          All kernels give bit-identical results. */
 int rsh_split_sizes(int32_t form, int32_t faces, int64_t n_cells, int32_t m, int64_t* state_bytes,
                     int64_t* scale_bytes);
 int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64_t n_cells, int32_t m,
                        void* d_state, float* d_cell_scale, void* stream);
+int rsh_pack_split_avg_dev(int32_t form, const double* d_illum, int64_t n_cells, int32_t m, void* d_state,
+                           float* d_cell_scale, void* stream);
 int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, const float* d_cell_scale,
                           int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
                           int32_t out_f32, void* stream, int32_t waves);
@@ -244,7 +253,9 @@ int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
  * (one per row-block class) captured as a hipGraph, the state stays resident,
  * and problems join and leave between runs (continuous batching).
  *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM)
- *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot;
+ *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot,
+ *                        ordered after all work submitted to the null stream
+ *                        (arrays written on other streams must be complete);
  *                        ds holds the directions and phase matrix; form selects
  *                        the FP64 state or a packed split state (RSH_FORM_BF16X2,
  *                        RSH_FORM_FP16X2, RSH_FORM_BF16X3; residual still FP64)

@@ -141,6 +141,7 @@ PROTOTYPES = {
     "rsh_problem_stream": (C.c_int, [vp, i32, P_vp]),
     "rsh_event_times": (C.c_int, [vp, P_f32]),
     "rsh_read_simple_bin": (C.c_int, [cp, i32, vp, i64, C.POINTER(i64)]),
+    "rsh_pack_split_avg_dev": (C.c_int, [i32, vp, i64, i32, vp, vp, vp]),
     "rsh_write_simple_bin": (C.c_int, [cp, i32, vp, i64]),
     "rsh_moments_f64_dev": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp]),
     "rsh_problem_set_geometry": (C.c_int, [vp, vp, vp, vp]),

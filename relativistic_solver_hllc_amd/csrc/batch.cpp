@@ -269,6 +269,16 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
     if (id < 0) return fail(RSH_E_NOMEM, "batch is full");
     RSH_HIP_TRY(hipSetDevice(b->device));
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    // The batch stream is non-blocking: order its copies of d_I0 / d_B after all
+    // work already submitted to the null stream (where callers usually made them).
+    {
+        hipEvent_t ready;
+        RSH_HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        hipError_t er = hipEventRecord(ready, nullptr);
+        if (er == hipSuccess) er = hipStreamWaitEvent(b->stream, ready, 0);
+        hipEventDestroy(ready);
+        if (er != hipSuccess) return hip_fail(er, "ordering batch_add after the null stream");
+    }
     Slot s;
     s.ds = ds;
     s.n = n_cells;

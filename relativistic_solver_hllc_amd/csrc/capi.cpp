@@ -61,9 +61,10 @@ void free_panel(PanelEntry& pe) {
 }
 
 // Panel for rows [k0,k1), face count and form, built on stream st if absent.
-int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out) {
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out,
+              const SplitPlan* resident) {
     std::lock_guard<std::mutex> lock(ds->mu);
-    auto key = std::make_tuple(k0, k1, faces, form);
+    auto key = std::make_tuple(k0, k1, faces, form, resident ? 1 : 0);
     auto it = ds->panels.find(key);
     if (it != ds->panels.end()) {
         // Make st wait for the build only while it may still be running.
@@ -87,7 +88,7 @@ int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st
         if (e == hipSuccess)
             e = launch_panel_f64(ds->d_P, ds->m, k0, k1, 1.0 / faces, pe.plan, static_cast<double*>(pe.d_panel), st);
     } else {
-        pe.splan = plan_split(ds->m, k1 - k0);
+        pe.splan = resident ? *resident : plan_split(ds->m, k1 - k0);
         const int64_t units = split_panel_units(form, pe.splan);
         e = hipMalloc(&pe.d_panel, std::max<int64_t>(units, 1) * 16);
         if (e == hipSuccess) e = hipMalloc(&pe.d_row_inv, std::max(pe.splan.rb_total * 16, 1) * sizeof(float));
@@ -158,14 +159,32 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int 
 
 int split_default_waves() { return 8; }
 
+}  // namespace
+
+namespace rsh {
+int device_cus(int device) {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(device);
+    if (it != cache.end()) return it->second;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    cache[device] = cus;
+    return cus;
+}
+}  // namespace rsh
+
+namespace {
+
 bool valid_split_form(int form) {
     return form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3;
 }
 
 int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_cell_inv, int faces, int64_t n,
                   int k0, int k1, void* d_out, int64_t ld, int out_f32, int waves, hipStream_t st) {
-    if (waves == 0) waves = split_default_waves();
-    if (waves != 4 && waves != 8) return fail(RSH_E_UNSUPPORTED, "split tile: waves must be 4 or 8");
+    if (waves != 0 && waves != 1 && waves != 4 && waves != 8)
+        return fail(RSH_E_UNSUPPORTED, "split kernel: 0 (auto), 1 (resident panel), 4 or 8 (streaming)");
     if (!ds) return fail(RSH_E_ARG, "null direction set");
     if (!valid_split_form(form)) return fail(RSH_E_ARG, "not a split form");
     if (faces != 1 && faces != 4) return fail(RSH_E_UNSUPPORTED, "faces must be 1 or 4");
@@ -174,7 +193,29 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     if (!d_state || !d_cell_inv || !d_out) return fail(RSH_E_ARG, "null device pointer");
     if (ld < n) return fail(RSH_E_ARG, "ld_out smaller than n_cells");
     RSH_HIP_TRY(hipSetDevice(ds->device));
+    SplitPlan rplan{};
+    int sets = 0;
+    const bool resident_ok = faces == 1 && plan_split_resident(form, ds->m, k1 - k0, device_cus(ds->device), &rplan, &sets);
+    if (waves == 1 && !resident_ok)
+        return fail(RSH_E_UNSUPPORTED, "resident-panel kernel needs cell-level state (faces = 1) and a panel slice "
+                                       "that fits LDS with at most one row group per CU of an XCD");
+    // Auto: the resident-panel kernel for two-part forms when its row groups
+    // keep at least 3/4 of the CUs busy (M = 800 needs 17 groups of 3 row
+    // blocks: 17 of 32 CUs per XCD). For bf16x3 the streaming kernel measured
+    // faster (4.2 vs 4.7 ms at 1.57 M cells x 400 directions).
+    const int per_xcd = device_cus(ds->device) / 8;
+    if (waves == 0)
+        waves = (resident_ok && split_terms(form) == 2 && 4 * sets * rplan.passes >= 3 * per_xcd)
+                    ? 1
+                    : split_default_waves();
     const PanelEntry* pe = nullptr;
+    if (waves == 1) {
+        int rc = get_panel(ds, k0, k1, faces, form, st, &pe, &rplan);
+        if (rc) return rc;
+        RSH_HIP_TRY(launch_split_resident(form, d_state, n, d_cell_inv, pe->d_panel, pe->d_row_inv, pe->splan, sets,
+                                          k1 - k0, d_out, ld, out_f32 ? 1 : 0, st));
+        return RSH_OK;
+    }
     int rc = get_panel(ds, k0, k1, faces, form, st, &pe);
     if (rc) return rc;
     RSH_HIP_TRY(launch_split_mfma(form, d_state, faces, n, d_cell_inv, pe->d_panel, pe->d_row_inv, pe->splan,
@@ -427,6 +468,17 @@ int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64
     hipStream_t st = (hipStream_t)stream;
     RSH_HIP_TRY(launch_cell_scale(d_illum, nullptr, n_cells, m, faces, d_cell_scale, st));
     RSH_HIP_TRY(launch_pack_state(form, d_illum, faces, n_cells, m, d_cell_scale, d_state, st));
+    return RSH_OK;
+}
+
+int rsh_pack_split_avg_dev(int32_t form, const double* d_illum, int64_t n_cells, int32_t m, void* d_state,
+                           float* d_cell_scale, void* stream) {
+    if (!valid_split_form(form) || n_cells < 0 || m <= 0) return fail(RSH_E_ARG, "invalid split pack arguments");
+    if (n_cells == 0) return RSH_OK;
+    if (!d_illum || !d_state || !d_cell_scale) return fail(RSH_E_ARG, "null device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    RSH_HIP_TRY(launch_cell_scale(d_illum, nullptr, n_cells, m, 4, d_cell_scale, st));
+    RSH_HIP_TRY(launch_pack_state_avg(form, d_illum, n_cells, m, d_cell_scale, d_state, st));
     return RSH_OK;
 }
 

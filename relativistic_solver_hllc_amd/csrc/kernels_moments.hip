@@ -29,17 +29,36 @@ namespace rsh {
 
 typedef double v2d_m __attribute__((ext_vector_type(2)));
 
+// 128-thread workgroups and four rows in flight per thread measured fastest
+// (83 % of the 8 TB/s HBM peak at 1.57 M cells x 400 directions, against 88 %
+// for a pure streaming read of the same bytes).
+constexpr int kMomentBlock = 128;
+constexpr int kMomentPipe = 4;
+
+// Non-temporal loads: every face value is read exactly once, so it should not
+// displace anything in the caches.
 template <int FPT>
 __device__ inline void load_faces(const double* __restrict__ p, double (&v)[FPT]) {
     if constexpr (FPT == 4) {
-        const v2d_m a = *reinterpret_cast<const v2d_m*>(p);
-        const v2d_m b = *reinterpret_cast<const v2d_m*>(p + 2);
+        const v2d_m a = __builtin_nontemporal_load(reinterpret_cast<const v2d_m*>(p));
+        const v2d_m b = __builtin_nontemporal_load(reinterpret_cast<const v2d_m*>(p + 2));
         v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
     } else if constexpr (FPT == 2) {
-        const v2d_m a = *reinterpret_cast<const v2d_m*>(p);
+        const v2d_m a = __builtin_nontemporal_load(reinterpret_cast<const v2d_m*>(p));
         v[0] = a.x; v[1] = a.y;
     } else {
-        v[0] = *p;
+        v[0] = __builtin_nontemporal_load(p);
+    }
+}
+
+template <int FPT>
+__device__ inline void accumulate(const double* __restrict__ ps, const double (&v)[FPT],
+                                  double (&acc)[FPT][kMomentFuncs]) {
+#pragma unroll
+    for (int q = 0; q < kMomentFuncs; q++) {
+        const double w = ps[q];
+#pragma unroll
+        for (int f = 0; f < FPT; f++) acc[f][q] = __builtin_fma(w, v[f], acc[f][q]);
     }
 }
 
@@ -50,14 +69,14 @@ __device__ inline double cell_sum(double x) {
     return x;
 }
 
-template <int FACES, int FPT>
-__global__ __launch_bounds__(256) void moments_f64_kernel(
+template <int FACES, int FPT, int PIPE>
+__global__ __launch_bounds__(kMomentBlock) void moments_f64_kernel(
     const double* __restrict__ illum, int64_t n, int m, const double* __restrict__ psi,
     const double* __restrict__ normals, const double* __restrict__ areas, const double* __restrict__ volume,
     double* __restrict__ energy, double* __restrict__ stream, double* __restrict__ impuls,
     double* __restrict__ divstream, double* __restrict__ divimpuls) {
     constexpr int TPC = FACES / FPT;
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t gid = (int64_t)blockIdx.x * kMomentBlock + threadIdx.x;
     const int64_t cell = gid / TPC;
     const int fg = (int)(gid % TPC);
     const bool active = cell < n;
@@ -73,18 +92,26 @@ __global__ __launch_bounds__(256) void moments_f64_kernel(
 #pragma unroll
         for (int q = 0; q < kMomentFuncs; q++) acc[f][q] = 0.0;
 
-#pragma unroll 4
-    for (int j = 0; j < m; j++) {
-        double v[FPT];
-        load_faces<FPT>(src + (int64_t)j * stride, v);
-        const double* __restrict__ ps = psi + j * kMomentFuncs;
+    // Ring of PIPE direction rows in flight: row j + PIPE is requested while
+    // row j is consumed (indices past the end re-read the last row, in bounds).
+    double ring[PIPE][FPT];
 #pragma unroll
-        for (int q = 0; q < kMomentFuncs; q++) {
-            const double w = ps[q];
+    for (int u = 0; u < PIPE; u++) load_faces<FPT>(src + (int64_t)(u < m ? u : m - 1) * stride, ring[u]);
+    int j = 0;
+    for (; j + PIPE <= m; j += PIPE) {
 #pragma unroll
-            for (int f = 0; f < FPT; f++) acc[f][q] = __builtin_fma(w, v[f], acc[f][q]);
+        for (int u = 0; u < PIPE; u++) {
+            double v[FPT];
+#pragma unroll
+            for (int f = 0; f < FPT; f++) v[f] = ring[u][f];
+            const int jn = j + u + PIPE;
+            load_faces<FPT>(src + (int64_t)(jn < m ? jn : m - 1) * stride, ring[u]);
+            accumulate<FPT>(psi + (j + u) * kMomentFuncs, v, acc);
         }
     }
+#pragma unroll
+    for (int u = 0; u < PIPE; u++)
+        if (j + u < m) accumulate<FPT>(psi + (j + u) * kMomentFuncs, ring[u], acc);
 
     double e = 0.0, s[3] = {0.0, 0.0, 0.0}, t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double ds = 0.0, di[3] = {0.0, 0.0, 0.0};
@@ -148,10 +175,10 @@ hipError_t launch_moments_f64(const double* d_illum, int faces, int64_t n, int m
     if (n <= 0) return hipSuccess;
     const int fpt = faces == 1 ? 1 : faces_per_thread;
     const int64_t threads = n * (faces / fpt);
-    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-#define RSH_MOM(FA, FP)                                                                                      \
-    hipLaunchKernelGGL((moments_f64_kernel<FA, FP>), grid, block, 0, st, d_illum, n, m, d_psi, d_normals,   \
-                       d_areas, d_volume, d_energy, d_stream, d_impuls, d_divstream, d_divimpuls)
+    const dim3 grid((unsigned)((threads + kMomentBlock - 1) / kMomentBlock)), block(kMomentBlock);
+#define RSH_MOM(FA, FP)                                                                                        \
+    hipLaunchKernelGGL((moments_f64_kernel<FA, FP, kMomentPipe>), grid, block, 0, st, d_illum, n, m, d_psi,   \
+                       d_normals, d_areas, d_volume, d_energy, d_stream, d_impuls, d_divstream, d_divimpuls)
     if (faces == 1) RSH_MOM(1, 1);
     else if (fpt == 4) RSH_MOM(4, 4);
     else if (fpt == 2) RSH_MOM(4, 2);

@@ -98,6 +98,41 @@ __global__ __launch_bounds__(256) void pack_state_kernel(const double* __restric
         for (int f = 0; f < F; f++) out[(t * F + f) * 64] = __builtin_bit_cast(v4u, buf[t][f]);
 }
 
+// Face-averaged pack: reads the 4 faces of (direction, cell), averages them in
+// FP64 (the sum of four doubles, times 1/4 exactly) and writes one cell-level
+// vector per part.
+template <int FMT, int NT>
+__global__ __launch_bounds__(256) void pack_state_avg_kernel(const double* __restrict__ illum, int64_t n, int m,
+                                                             int steps, const float* __restrict__ inv_scale,
+                                                             v4u* __restrict__ state, int64_t total) {
+    typedef typename Fmt<FMT>::T T;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx & 63);
+    const int64_t frag = idx >> 6;
+    const int s = (int)(frag % steps);
+    const int64_t ct = frag / steps;
+    const int64_t cell = ct * 16 + (lane & 15);
+    const int j0 = 32 * s + 8 * (lane >> 4);
+    const bool cvalid = cell < n;
+    const double scale = cvalid ? 1.0 / (double)inv_scale[cell] : 0.0;
+    T buf[NT][8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int j = j0 + e;
+        const bool ok = cvalid && j < m;
+        const double* p = illum + ((int64_t)(ok ? j : 0) * n + (ok ? cell : 0)) * 4;
+        const float x = ok ? (float)((((p[0] + p[1]) + p[2]) + p[3]) * 0.25 * scale) : 0.0f;
+        T parts[NT];
+        split_parts<FMT, NT>(x, parts);
+#pragma unroll
+        for (int t = 0; t < NT; t++) buf[t][e] = parts[t];
+    }
+    v4u* out = state + (frag * NT) * 64 + lane;
+#pragma unroll
+    for (int t = 0; t < NT; t++) out[t * 64] = __builtin_bit_cast(v4u, buf[t]);
+}
+
 // ------------------------------------------------------------------ phase panel
 // One block per output row: row scale from the row maximum, then the fragments.
 template <int FMT, int NT>
@@ -231,6 +266,23 @@ hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t
     else if (fmt == 1 && nt == 2) { if (faces == 4) RSH_PACK(1, 2, 4); else RSH_PACK(1, 2, 1); }
     else { if (faces == 4) RSH_PACK(0, 3, 4); else RSH_PACK(0, 3, 1); }
 #undef RSH_PACK
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_state_avg(int form, const double* d_illum, int64_t n, int m, const float* d_inv,
+                                 void* d_state, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int steps = (m + 31) / 32;
+    const int64_t total = (n + 15) / 16 * steps * 64;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    v4u* out = static_cast<v4u*>(d_state);
+    const int fmt = split_fmt(form), nt = split_terms(form);
+    if (fmt == 0 && nt == 2)
+        hipLaunchKernelGGL((pack_state_avg_kernel<0, 2>), grid, block, 0, st, d_illum, n, m, steps, d_inv, out, total);
+    else if (fmt == 1 && nt == 2)
+        hipLaunchKernelGGL((pack_state_avg_kernel<1, 2>), grid, block, 0, st, d_illum, n, m, steps, d_inv, out, total);
+    else
+        hipLaunchKernelGGL((pack_state_avg_kernel<0, 3>), grid, block, 0, st, d_illum, n, m, steps, d_inv, out, total);
     return hipGetLastError();
 }
 

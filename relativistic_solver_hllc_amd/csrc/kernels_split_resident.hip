@@ -1,0 +1,263 @@
+// Split-precision contraction with the phase panel resident in LDS
+// ("resident-panel" kernel), for cell-level state (one value per cell and
+// direction, i.e. face-averaged intensities).
+//
+// The streaming kernel (kernels_split.hip) re-streams the whole panel through
+// LDS for every 64 cells and synchronises its workgroup at every K-step; at
+// 100 KB of LDS per workgroup only one workgroup fits a CU, so those barriers
+// are exposed. Here each workgroup owns a fixed group of RBW row blocks (16 RBW
+// output directions) and copies that slice of the panel for ALL K-steps into
+// LDS once (RBW * steps * NT KB, <= 160 KB), then streams cells through it with
+// no further barrier: persistent workgroups, one per CU.
+//
+// Work split. The G row groups of a problem are served by G workgroups that
+// sit on the same XCD (workgroup b runs on XCD b % 8) and walk the same
+// sequence of 256-cell chunks, so a chunk's state is read from HBM once and
+// from that XCD's L2 by the other G - 1. Each XCD owns a contiguous 1/8 of the
+// chunks; within it, the floor(CUs per XCD / G) sets of G workgroups take
+// chunks round-robin.
+//
+// Per wave: two 16-cell tiles x RBW row blocks, so every A fragment read from
+// LDS feeds two MFMAs per product; MFMAs are issued product-major so that no
+// MFMA depends on the one before it; state vectors are prefetched two K-steps
+// ahead with ordinary (L2-allocating) loads, since the other row groups of the
+// XCD read the same vectors. Products and their order per (row block, tile) are
+// those of the streaming kernel (main hi*hi; cross hi*lo + lo*hi, and for three
+// parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi), so the kernels give
+// bit-identical results on the same state and panel.
+#include "mfma_split_core.hpp"
+
+namespace rsh {
+
+namespace {
+
+constexpr int kResWaves = 8;   // waves per workgroup (2 per SIMD)
+constexpr int kResTiles = 2;   // 16-cell tiles per wave
+constexpr int kResXd = 2;      // state prefetch depth (K-steps ahead)
+constexpr int kResChunkTiles = kResWaves * kResTiles;  // 256 cells per chunk
+constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kResLdsBytes = 160 * 1024;
+
+// STEPS > 0: the K-step count is a compile-time constant and the K loop is
+// fully unrolled, so the state prefetch crosses no loop back-edge (where the
+// compiler would otherwise wait for every outstanding load). STEPS = 0: any M.
+template <int RBW, int FMT, int NT, int STEPS>
+__global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
+    const v4u* __restrict__ state, int64_t n, int steps_rt, const float* __restrict__ cell_inv,
+    const v4u* __restrict__ panel, const float* __restrict__ row_inv, void* __restrict__ out, int64_t ld, int rows,
+    int out_f32, int groups, int sets, int64_t chunks) {
+    typedef typename Fmt<FMT>::V V;
+    constexpr int XD = kResXd;
+    const int steps = STEPS > 0 ? STEPS : steps_rt;
+    extern __shared__ v4u lds[];  // [steps][RBW][NT][64]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int xcd = blockIdx.x % kXcds;
+    const int local = blockIdx.x / kXcds;
+    const int set = local / groups;
+    const int rg = local % groups;
+    if (set >= sets) return;  // whole workgroup leaves together: no barrier below is skipped by part of it
+
+    // 1. the row group's panel slice, all K-steps, into LDS (contiguous in the panel)
+    const int64_t units = (int64_t)steps * RBW * NT * 64;
+    const v4u* __restrict__ src = panel + (int64_t)rg * units;
+    for (int64_t u = threadIdx.x; u < units; u += 64 * kResWaves) lds[u] = src[u];
+    const int row0 = rg * RBW * 16;
+    __syncthreads();
+
+    // 2. chunks of this XCD, round-robin over its sets
+    const int64_t c_begin = chunks * xcd / kXcds, c_end = chunks * (xcd + 1) / kXcds;
+    const int64_t tiles = (n + 15) / 16;
+    for (int64_t chunk = c_begin + set; chunk < c_end; chunk += sets) {
+        const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
+        // Tiles past the end read the last tile (in bounds) and are not written:
+        // unconditional loads keep the prefetch visible to the wait counters.
+        const v4u* xsrc[kResTiles];
+#pragma unroll
+        for (int q = 0; q < kResTiles; q++)
+            xsrc[q] = state + (ct0 + q < tiles ? ct0 + q : tiles - 1) * (int64_t)steps * NT * 64 + lane;
+        v4u xr[XD][kResTiles][NT];
+        auto load_x = [&](auto slot_tag, int t) {
+            constexpr int sl = decltype(slot_tag)::value;
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++)
+#pragma unroll
+                for (int p = 0; p < NT; p++) xr[sl][q][p] = xsrc[q][((int64_t)t * NT + p) * 64];
+        };
+        v4f accm[RBW][kResTiles], accx[RBW][kResTiles];
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) {
+                accm[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
+                accx[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        auto kstep = [&](auto slot_tag, const int t) {
+            constexpr int sl = decltype(slot_tag)::value;
+            // Scheduling fence per K-step: in the unrolled loop the scheduler
+            // would otherwise hoist every K-step's loads and run out of registers.
+            __builtin_amdgcn_sched_barrier(0);
+            V bx[kResTiles][NT];
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++)
+#pragma unroll
+                for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
+            if (t + XD < steps) load_x(slot_tag, t + XD);
+            const v4u* a_base = lds + (int64_t)t * RBW * NT * 64 + lane;
+            // A fragments of one part at a time (part-major), so that only RBW
+            // fragments are live. Product-major order: each round touches every
+            // (row block, tile) accumulator once, so an MFMA never waits for the
+            // previous result of its own accumulator.
+            V a[RBW];
+            auto fetch = [&](int p) {
+#pragma unroll
+                for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NT + p) * 64]);
+            };
+            auto round = [&](v4f (&acc)[RBW][kResTiles], int pb) {
+#pragma unroll
+                for (int i = 0; i < RBW; i++)
+#pragma unroll
+                    for (int q = 0; q < kResTiles; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
+            };
+            fetch(0);
+            round(accm, 0);
+            if constexpr (NT == 2) {
+                round(accx, 1);
+                fetch(1);
+                round(accx, 0);
+            } else {
+                round(accx, 2);
+                round(accx, 1);
+                fetch(1);
+                round(accx, 1);
+                round(accx, 0);
+                fetch(2);
+                round(accx, 0);
+            }
+        };
+        load_x(std::integral_constant<int, 0>{}, 0);
+        if (steps > 1) load_x(std::integral_constant<int, 1>{}, 1);
+        if constexpr (STEPS > 0) {
+#pragma unroll
+            for (int t = 0; t < STEPS; t += 2) {
+                kstep(std::integral_constant<int, 0>{}, t);
+                if (t + 1 < STEPS) kstep(std::integral_constant<int, 1>{}, t + 1);
+            }
+        } else {
+            for (int t = 0; t < steps; t += 2) {
+                kstep(std::integral_constant<int, 0>{}, t);
+                if (t + 1 < steps) kstep(std::integral_constant<int, 1>{}, t + 1);
+            }
+        }
+        // epilogue: FP64 combination of main and cross terms, exact scale removal
+#pragma unroll
+        for (int q = 0; q < kResTiles; q++) {
+            const int64_t cell = (ct0 + q) * 16 + (lane & 15);
+            if (ct0 + q >= tiles || cell >= n) continue;
+            const double cinv = (double)cell_inv[cell];
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = row0 + i * 16 + 4 * g + r;
+                    if (row < rows) {
+                        const double v =
+                            ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)row_inv[row] * cinv;
+                        if (out_f32)
+                            __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
+                        else
+                            __builtin_nontemporal_store(v, static_cast<double*>(out) + (int64_t)row * ld + cell);
+                    }
+                }
+        }
+    }
+}
+
+template <int RBW, int FMT, int NT, int STEPS>
+hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* state, int64_t n, int steps,
+                      const float* cell_inv, const v4u* panel, const float* row_inv, void* out, int64_t ld, int rows,
+                      int out_f32, int groups, int sets, int64_t chunks) {
+    auto kern = split_resident_kernel<RBW, FMT, NT, STEPS>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kResWaves), lds_bytes, st, state, n, steps, cell_inv, panel,
+                       row_inv, out, ld, rows, out_f32, groups, sets, chunks);
+    return hipGetLastError();
+}
+
+// Largest row-block count per workgroup that the compiler keeps in 256
+// registers without spills (two parts: 5; three parts: 4).
+constexpr int res_max_rbw(int nt) { return nt == 2 ? 5 : 4; }
+constexpr int kResMaxRbw = 5;
+
+}  // namespace
+
+// Row-group plan of the resident-panel kernel for output rows [0, rows) of an
+// M-direction set: rb_per_pass = RBW row blocks per workgroup (largest slice
+// that fits LDS, then balanced), passes = G row groups. False if the kernel
+// does not apply (G row groups do not fit one XCD's CUs).
+bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets) {
+    const int nt = split_terms(form);
+    const int steps = (m + 31) / 32;
+    const int rb_need = (rows + 15) / 16;
+    const int per_rb = steps * nt * 1024;
+    int rbw_max = kResLdsBytes / per_rb;
+    if (rbw_max > res_max_rbw(nt)) rbw_max = res_max_rbw(nt);
+    if (rbw_max < 1 || rb_need < 1) return false;
+    const int groups = (rb_need + rbw_max - 1) / rbw_max;
+    const int rbw = (rb_need + groups - 1) / groups;
+    const int per_xcd = cus / kXcds;
+    if (groups > per_xcd) return false;
+    pl->rb_per_pass = rbw;
+    pl->passes = groups;
+    pl->rb_total = rbw * groups;
+    pl->steps = steps;
+    *sets = per_xcd / groups;
+    return true;
+}
+
+hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const float* d_cell_inv,
+                                 const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int sets, int rows,
+                                 void* d_out, int64_t ld, int out_f32, hipStream_t st) {
+    if (n <= 0 || rows <= 0) return hipSuccess;
+    const int nt = split_terms(form), fmt = split_fmt(form);
+    const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * nt * 1024;
+    if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > res_max_rbw(nt) || sets < 1) return hipErrorInvalidValue;
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
+    const int blocks = kXcds * sets * pl.passes;
+    const v4u* state = static_cast<const v4u*>(d_state);
+    const v4u* panel = static_cast<const v4u*>(d_panel);
+#define RSH_RES_S(RBW, FMT, NT, S)                                                                              \
+    return launch_res<RBW, FMT, NT, S>(blocks, lds_bytes, st, state, n, pl.steps, d_cell_inv, panel, d_row_inv, \
+                                       d_out, ld, rows, out_f32, pl.passes, sets, chunks)
+    // Unrolled instances for the plans of M = 200, 400, 800 (7, 13, 25 K-steps).
+#define RSH_RES(RBW, FMT, NT)                                                                          \
+    {                                                                                                  \
+        if constexpr (NT == 2 && RBW == 5) { if (pl.steps == 7) RSH_RES_S(RBW, FMT, NT, 7); }          \
+        if constexpr (NT == 2 && RBW == 5) { if (pl.steps == 13) RSH_RES_S(RBW, FMT, NT, 13); }        \
+        if constexpr (NT == 2 && RBW == 3) { if (pl.steps == 25) RSH_RES_S(RBW, FMT, NT, 25); }        \
+        if constexpr (NT == 3 && RBW == 4) { if (pl.steps == 7) RSH_RES_S(RBW, FMT, NT, 7); }          \
+        if constexpr (NT == 3 && RBW == 4) { if (pl.steps == 13) RSH_RES_S(RBW, FMT, NT, 13); }        \
+        if constexpr (NT == 3 && RBW == 2) { if (pl.steps == 25) RSH_RES_S(RBW, FMT, NT, 25); }        \
+        RSH_RES_S(RBW, FMT, NT, 0);                                                                    \
+    }
+#define RSH_RES_FORMS(RBW)                                  \
+    case RBW:                                               \
+        if (fmt == 0 && nt == 2) RSH_RES(RBW, 0, 2);        \
+        if (fmt == 1 && nt == 2) RSH_RES(RBW, 1, 2);        \
+        if constexpr (RBW <= res_max_rbw(3)) RSH_RES(RBW, 0, 3); \
+        return hipErrorInvalidValue;
+    switch (pl.rb_per_pass) {
+        RSH_RES_FORMS(1) RSH_RES_FORMS(2) RSH_RES_FORMS(3) RSH_RES_FORMS(4) RSH_RES_FORMS(5)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef RSH_RES_FORMS
+#undef RSH_RES
+#undef RSH_RES_S
+}
+
+}  // namespace rsh

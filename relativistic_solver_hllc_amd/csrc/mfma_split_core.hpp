@@ -175,11 +175,13 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
                 accx[i] = Fmt<FMT>::mfma(a[0], bx[1], accx[i]);
                 accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
             } else {
-                accx[i] = Fmt<FMT>::mfma(a[1], bx[1], accx[i]);
+                // part-major order (hi*lo, hi*mid, mid*mid, mid*hi, lo*hi), shared
+                // with the resident-panel kernel
                 accx[i] = Fmt<FMT>::mfma(a[0], bx[2], accx[i]);
-                accx[i] = Fmt<FMT>::mfma(a[2], bx[0], accx[i]);
                 accx[i] = Fmt<FMT>::mfma(a[0], bx[1], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[1], bx[1], accx[i]);
                 accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
+                accx[i] = Fmt<FMT>::mfma(a[2], bx[0], accx[i]);
             }
         }
 #undef RSH_AFRAG

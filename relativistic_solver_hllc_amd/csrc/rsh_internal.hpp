@@ -75,6 +75,14 @@ hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t
                              void* d_state, hipStream_t st);
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
                               const SplitPlan& pl, void* d_panel, float* d_row_inv, hipStream_t st);
+// Resident-panel kernel (cell-level state only): plan and launch.
+bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets);
+hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const float* d_cell_inv,
+                                 const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int sets, int rows,
+                                 void* d_out, int64_t ld, int out_f32, hipStream_t st);
+// Packs the face average of 4-face FP64 input as a cell-level state.
+hipError_t launch_pack_state_avg(int form, const double* d_illum, int64_t n, int m, const float* d_inv,
+                                 void* d_state, hipStream_t st);
 bool split_tile_supported(int waves, int row_split);
 hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n, const float* d_cell_inv,
                              const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int rows,

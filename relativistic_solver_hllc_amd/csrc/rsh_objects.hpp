@@ -29,14 +29,18 @@ struct rsh_dirset_s {
     double* d_P = nullptr;  // [m][m] FP64 phase matrix
     double* d_psi = nullptr;  // [m][kMomentFuncs] moment weights (full_area > 0 only)
     std::mutex mu;
-    // key: (k0, k1, faces, form)
-    std::map<std::tuple<int, int, int, int>, PanelEntry> panels;
+    // key: (k0, k1, faces, form, layout); layout 0 = streaming plan, 1 = resident-panel plan
+    std::map<std::tuple<int, int, int, int, int>, PanelEntry> panels;
     bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
 };
 
 namespace rsh {
-// Phase panel of (k0, k1, faces, form), built on stream st if absent.
-int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out);
+// Phase panel of (k0, k1, faces, form), built on stream st if absent. A split
+// panel with resident != nullptr uses that row-group plan (layout 1).
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out,
+              const SplitPlan* resident = nullptr);
+// Compute units of a device (cached).
+int device_cus(int device);
 // Tile of the FP64 MFMA kernel chosen for a row-block class (tuned or default).
 rsh_tile f64_tile_for(int device, int rb);
 void set_tuned_tile(int device, int rb, rsh_tile t);

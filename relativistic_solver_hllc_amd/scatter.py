@@ -136,6 +136,12 @@ def pack_split_dev(form: str, illum, faces: int, n_cells: int, m: int, state, ce
 
 # ------------------------------------------------------------------ direction set
 
+def pack_split_avg_dev(form: str, illum, n_cells: int, m: int, state, cell_scale, stream=None) -> None:
+    """Cell-level split state (faces = 1) of the face average of 4-face FP64 illum."""
+    N.call("rsh_pack_split_avg_dev", FORMS[form], _ptr(illum), n_cells, m, _ptr(state), _ptr(cell_scale),
+           _stream(stream))
+
+
 class DirectionSet:
     """Directions, weights and the device FP64 phase matrix of one direction set."""
 

@@ -118,3 +118,75 @@ def test_split_tiles_bit_identical(gpu, form):
         for a in (d_illum, state, scale):
             a.free()
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
+@pytest.mark.parametrize("n,m,k0,k1", [(70001, 400, 0, 400), (5000, 200, 0, 200), (3000, 400, 37, 301),
+                                       (17, 26, 0, 26), (9000, 800, 0, 800)])
+def test_resident_panel_kernel_bit_identical(gpu, form, n, m, k0, k1):
+    """Cell-level state: the resident-panel kernel (waves=1, the default for faces = 1)
+    and both streaming kernels sum the same products in the same order."""
+    rng = np.random.default_rng(n + m + k0)
+    cell = rng.lognormal(0.0, 1.5, size=(m, n, 1))
+    xyz, w, F = rs.fibonacci_directions(m)
+    with rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True) as ds:
+        d_cell = D.DeviceArray.from_host(cell)
+        sb, cb = rs.split_sizes(form, 1, n, m)
+        state, scale = D.DeviceArray((sb,), np.uint8), D.DeviceArray((cb // 4,), np.float32)
+        rs.pack_split_dev(form, d_cell, 1, n, m, state, scale)
+        outs = {}
+        for waves in (1, 0, 4, 8):
+            out = D.DeviceArray((k1 - k0, n + 3))
+            out.fill(-1.0)
+            ds.scatter_split(form, state, scale, 1, n, k0, k1, out, ld=n + 3, waves=waves)
+            D.synchronize()
+            outs[waves] = out.to_host()
+            out.free()
+        for a in (d_cell, state, scale):
+            a.free()
+    ref = outs[4]
+    assert np.all(ref[:, n:] == -1.0)  # padding columns untouched
+    for wv, o in outs.items():
+        assert np.array_equal(o, ref), wv
+    if m <= 400 and n <= 5000:
+        P = orc.phase_matrix(xyz, w, F, 1, 0.8, True, k0, k1)
+        exact = orc.scatter_matrix(cell, P, threads=16)
+        mx, mean = errs(ref[:, :n], exact)
+        assert mx <= TOL[form][0] and mean <= TOL[form][1], (mx, mean)
+
+
+@pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
+def test_face_averaged_pack_vs_oracle(gpu, form):
+    """rsh_pack_split_avg_dev: 4-face input reduced to a cell-level state in the pack;
+    the resident-panel kernel on it matches the FP64 oracle on the 4-face input."""
+    n, m = 4096, 400
+    rng = np.random.default_rng(0x5EED + 33)
+    illum = rng.lognormal(0.0, 1.5, size=(m, n, 4))
+    xyz, w, F = rs.fibonacci_directions(m)
+    ref = orc.scatter_matrix(illum, orc.phase_matrix(xyz, w, F, 1, 0.8, True), threads=16)
+    with rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True) as ds:
+        d_illum = D.DeviceArray.from_host(illum)
+        sb, cb = rs.split_sizes(form, 1, n, m)
+        state, scale = D.DeviceArray((sb,), np.uint8), D.DeviceArray((cb // 4,), np.float32)
+        rs.pack_split_avg_dev(form, d_illum, n, m, state, scale)
+        out = D.DeviceArray((m, n))
+        ds.scatter_split(form, state, scale, 1, n, 0, m, out)
+        D.synchronize()
+        got = out.to_host()
+        for a in (d_illum, state, scale, out):
+            a.free()
+    mx, mean = errs(got, ref)
+    print(f"{form} face-averaged pack: max rel err {mx:.3e}, mean {mean:.3e}")
+    assert mx <= TOL[form][0] and mean <= TOL[form][1]
+
+
+def test_resident_kernel_rejects_face_state(gpu):
+    xyz, w, F = rs.fibonacci_directions(26)
+    with rs.DirectionSet(xyz, w, F) as ds:
+        sb, cb = rs.split_sizes("bf16x2", 4, 64, 26)
+        state, scale = D.DeviceArray((sb,), np.uint8), D.DeviceArray((cb // 4,), np.float32)
+        out = D.DeviceArray((26, 64))
+        with pytest.raises(RuntimeError, match="resident-panel"):
+            ds.scatter_split("bf16x2", state, scale, 4, 64, 0, 26, out, waves=1)
+        for a in (state, scale, out):
+            a.free()
