@@ -15,6 +15,9 @@ Prints ONE JSON line. Extra keys:
   cpu_baseline  the oracle's restatement of the reference CPU path
                 (GetS + CalculateIntCPU), timed on this host on a slice of
                 output directions over all cells, scaled to a full evaluation
+  moments       radiation moments of the same intensities (HBM roofline)
+  split         split-precision forms (HG g = 0.8 on the same intensities)
+  batch, serve  batched source iteration of uneven problems; serving trace
 """
 from __future__ import annotations
 
@@ -89,37 +92,121 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
     P = orc.phase_matrix(xyz, w, F, 1, 0.8, True)
     ref_or = orc.scatter_matrix(illum.gather_cells(cells), P, threads=16)
     log(f"[bench] config3 FP64 HG: {t64:.3f} ms")
-    for form, faces_note in (("bf16x2", "faces"), ("fp16x2", "faces"), ("bf16x3", "faces")):
+    out32 = D.DeviceArray((m, n), np.float32)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        t_ms = e0.elapsed_ms(e1) / steps
+        stream.synchronize()
+        return t_ms
+
+    def errors(o):
+        got_or = o.gather_columns(cells).astype(np.float64)
+        r_or = np.abs(got_or - ref_or) / ref_or
+        got = o.gather_columns(wide).astype(np.float64)
+        r = np.abs(got - ref64) / ref64
+        return {"max_rel_err_vs_oracle": float(r_or.max()), "mean_rel_err_vs_oracle": float(r_or.mean()),
+                "max_rel_err_vs_fp64_gpu": float(r.max()), "mean_rel_err_vs_fp64_gpu": float(r.mean())}
+
+    # Resident split state: the face average packed once (as the source
+    # iteration keeps it), contracted by the default split kernel.
+    for form in ("bf16x2", "fp16x2", "bf16x3"):
+        sb, cb = rs.split_sizes(form, 1, n, m)
+        state = D.DeviceArray((sb,), np.uint8)
+        scale = D.DeviceArray((cb // 4,), np.float32)
+        e0.record(stream)
+        rs.pack_split_avg_dev(form, illum, n, m, state, scale, stream=stream)
+        e1.record(stream)
+        pack_ms = e0.elapsed_ms(e1)
+        entry = {"state": "cell-level (face average packed once)", "state_bytes": sb, "pack_ms": pack_ms,
+                 "kernel": "resident-panel" if form != "bf16x3" else "streaming-panel (8 waves)"}
+        for label, o, ob in (("fp64_out", out, 8), ("fp32_out", out32, 4)):
+            t = timed(lambda: ds.scatter_split(form, state, scale, 1, n, 0, m, o, out_f32=(ob == 4), stream=stream))
+            byts = sb + n * m * ob
+            entry[label] = {"ms": t, "speedup_vs_fp64": t64 / t, "pairs_per_s": n * m / (t * 1e-3),
+                            "hbm_gbs_algorithmic": byts / (t * 1e-3) / 1e9,
+                            "roofline_frac_hbm": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": byts,
+                            **errors(o)}
+            log(f"[bench] {form} cell state {label}: {t:.3f} ms ({t64 / t:.2f}x FP64), "
+                f"max rel err vs oracle {entry[label]['max_rel_err_vs_oracle']:.2e}")
+        entry["err_sample"] = f"{len(cells)} cells x {m} dirs vs oracle; {len(wide)} cells x {m} dirs vs FP64 GPU"
+        res["forms"][form] = entry
+        state.free()
+        scale.free()
+    # Face-level split state (4 faces per cell kept apart, summed in the kernel).
+    res["face_state"] = {}
+    for form in ("bf16x2", "fp16x2", "bf16x3"):
         sb, cb = rs.split_sizes(form, 4, n, m)
         state = D.DeviceArray((sb,), np.uint8)
         scale = D.DeviceArray((cb // 4,), np.float32)
         rs.pack_split_dev(form, illum, 4, n, m, state, scale, stream=stream)
-        for _ in range(warmup):
-            ds.scatter_split(form, state, scale, 4, n, 0, m, out, stream=stream)
-        e0.record(stream)
-        for _ in range(steps):
-            ds.scatter_split(form, state, scale, 4, n, 0, m, out, stream=stream)
-        e1.record(stream)
-        t = e0.elapsed_ms(e1) / steps
-        stream.synchronize()
-        got_or = out.gather_columns(cells)
-        r_or = np.abs(got_or - ref_or) / ref_or
-        got = out.gather_columns(wide)
-        r = np.abs(got - ref64) / ref64
+        t = timed(lambda: ds.scatter_split(form, state, scale, 4, n, 0, m, out, stream=stream))
         byts = sb + n * m * 8
-        res["forms"][form] = {
-            "ms": t, "speedup_vs_fp64": t64 / t, "pairs_per_s": n * m / (t * 1e-3),
-            "hbm_gbs_algorithmic": byts / (t * 1e-3) / 1e9, "roofline_frac_hbm": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "bytes_per_launch": byts, "state_bytes": sb,
-            "max_rel_err_vs_oracle": float(r_or.max()), "mean_rel_err_vs_oracle": float(r_or.mean()),
-            "max_rel_err_vs_fp64_gpu": float(r.max()), "mean_rel_err_vs_fp64_gpu": float(r.mean()),
-            "err_sample": f"{len(cells)} cells x {m} dirs vs oracle; {len(wide)} cells x {m} dirs vs FP64 GPU",
-        }
-        log(f"[bench] {form}: {t:.3f} ms ({t64 / t:.2f}x FP64), max rel err {r.max():.2e} mean {r.mean():.2e}")
+        res["face_state"][form] = {"ms": t, "speedup_vs_fp64": t64 / t, "state_bytes": sb,
+                                   "hbm_gbs_algorithmic": byts / (t * 1e-3) / 1e9, **errors(out)}
+        log(f"[bench] {form} face state: {t:.3f} ms ({t64 / t:.2f}x FP64)")
         state.free()
         scale.free()
+    out32.free()
     ds.close()
     return res
+
+
+def moments_section(D, rs, ds, illum, xyz, w, F, n, m, steps, stream, cells):
+    """Radiation moments of the same resident face intensities (energy, stream,
+    impuls, divstream, divimpuls; the reference's CalculateAllParam), with
+    synthetic per-cell geometry; HBM-bound, so its roofline is the HBM peak."""
+    from oracle import orc
+
+    rng = np.random.default_rng(0x5EED + 6)
+    nrm = rng.normal(size=(n, 4, 3))
+    nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+    areas = rng.uniform(0.5, 1.5, size=(n, 4))
+    volume = rng.uniform(0.5, 1.5, size=n)
+    geo = [D.DeviceArray.from_host(a) for a in (nrm, areas, volume)]
+    width = {"energy": 1, "stream": 3, "impuls": 9, "divstream": 1, "divimpuls": 3}
+    outs = {k: D.DeviceArray((n, v)) for k, v in width.items()}
+
+    def go():
+        ds.moments(illum, 4, n, energy=outs["energy"], stream_out=outs["stream"], impuls=outs["impuls"],
+                   divstream=outs["divstream"], divimpuls=outs["divimpuls"], normals=geo[0], areas=geo[1],
+                   volume=geo[2], stream=stream)
+
+    go()
+    e0, e1 = D.Event(), D.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        go()
+    e1.record(stream)
+    t = e0.elapsed_ms(e1) / steps
+    stream.synchronize()
+    byts = 8 * (4 * m * n + 17 * n + 17 * n)  # faces read once, geometry read, five outputs written
+    ref = orc.moments(illum.gather_cells(cells), xyz, w, F, nrm[cells], areas[cells], volume[cells], threads=16)
+    sub = illum.gather_cells(cells)
+    e = np.max(np.einsum("j,jcf->cf", w, np.abs(sub)) / F, axis=1)
+    d = e * areas[cells].sum(axis=1) / volume[cells]
+    worst = 0.0
+    for k, v in width.items():
+        got = outs[k].to_host()[cells]
+        got = got[:, 0] if v == 1 else got
+        err = np.abs(got - ref[k])
+        err = err if err.ndim == 1 else err.max(axis=1)
+        worst = max(worst, float(np.max(err / (d if k.startswith("div") else e))))
+    for a in [*geo, *outs.values()]:
+        a.free()
+    log(f"[bench] moments: {t:.3f} ms, {byts / (t * 1e-3) / 1e9:.0f} GB/s, scaled max err {worst:.2e}")
+    return {"workload": "radiation moments (energy, stream, impuls, divstream, divimpuls) of the same "
+                        "1,572,864-cell x 4-face intensities, M=400, synthetic cell geometry",
+            "kernel": "moments_f64 (FP64 VALU, HBM-bound)", "ms": t, "cells_per_s": n / (t * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": byts / (t * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": byts},
+            "parity": {"sampled_cells": int(len(cells)), "max_scaled_err_vs_oracle": worst, "tolerance": 1e-12,
+                       "scale": "per cell: max_f sum_j w_j |I_jf| / F (x sum_f area_f / volume for divergences)"}}
 
 
 def config4_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
@@ -238,6 +325,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-slice", type=int, default=16)
+    ap.add_argument("--no-moments", action="store_true", help="skip the radiation-moments section")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision (config 3) section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems (config 4) section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) section")
@@ -315,6 +403,10 @@ def main() -> int:
         if pmc.get("n_cells") == n and pmc.get("n_dirs") == m:
             traffic = pmc.get("hbm_bytes_per_launch")
 
+    moments = None
+    if not args.no_moments:
+        moments = moments_section(D, rs, ds, illum, xyz, w, F, n, m, max(args.steps // 2, 2), stream, cells)
+
     split = None
     if not args.no_split:
         split = split_section(D, rs, illum, out, xyz, w, F, n, m, max(args.steps // 2, 2), 1, stream, cells)
@@ -372,6 +464,7 @@ def main() -> int:
         },
         "parity": {"sampled_cells": int(len(cells)), "max_rel_err_vs_oracle": parity, "tolerance": 1e-12},
         "cpu_baseline": cpu,
+        "moments": moments,
         "split": split,
         "batch": batch,
         "serve": serve,

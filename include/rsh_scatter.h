@@ -268,6 +268,17 @@ typedef struct rsh_batch_s* rsh_batch;
 int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes);
 int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_batch* out);
 int rsh_batch_destroy(rsh_batch b);
+/* Reserves one device arena of `bytes` for the batch's slot buffers (the batch
+ * must be empty): later adds carve their buffers from it first-fit instead of
+ * calling hipMalloc / hipFree (which synchronise the device). An add that finds
+ * no free block large enough returns RSH_E_NOMEM. bytes = 0 releases it. */
+int rsh_batch_reserve(rsh_batch b, int64_t bytes);
+/* Like rsh_batch_add, with I0 ~ log-normal(0, 1) (seed 2 seed + 1) and
+ * B ~ U(0.5, 2) (seed 2 seed + 2) generated on the device in the slot itself
+ * (rsh_fill_lognormal_dev / rsh_fill_uniform_dev); asynchronous, ordered on the
+ * batch stream. */
+int rsh_batch_add_seeded(rsh_batch b, rsh_dirset ds, int64_t n_cells, uint64_t seed, double albedo,
+                         int32_t form, int32_t* slot);
 int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B,
                   double albedo, int32_t form, int32_t* slot);
 int rsh_batch_remove(rsh_batch b, int32_t slot);

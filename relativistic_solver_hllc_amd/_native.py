@@ -96,6 +96,8 @@ PROTOTYPES = {
     "rsh_batch_create": (C.c_int, [i32, i32, i32, P_vp]),
     "rsh_batch_destroy": (C.c_int, [vp]),
     "rsh_batch_add": (C.c_int, [vp, vp, i64, vp, vp, f64, i32, P_i32]),
+    "rsh_batch_add_seeded": (C.c_int, [vp, vp, i64, u64, f64, i32, P_i32]),
+    "rsh_batch_reserve": (C.c_int, [vp, i64]),
     "rsh_batch_remove": (C.c_int, [vp, i32]),
     "rsh_batch_run": (C.c_int, [vp, i32]),
     "rsh_batch_sync": (C.c_int, [vp]),

@@ -16,6 +16,8 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -38,6 +40,7 @@ struct Slot {
     void* state[2] = {nullptr, nullptr}; // split forms
     float* cell_inv = nullptr;
     double* B = nullptr;
+    size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I, state, cell_inv
     F64Plan plan{};
     SplitPlan splan{};
     int iterations = 0;
@@ -74,17 +77,73 @@ struct rsh_batch_s {
     hipGraphExec_t exec = nullptr;
     hipStream_t stream = nullptr;
     int parity = 0;
+    // Optional device arena (rsh_batch_reserve): slot buffers are carved from
+    // it first-fit (256-byte aligned) instead of hipMalloc / hipFree, which
+    // synchronise the device and cost milliseconds at GB sizes.
+    char* arena = nullptr;
+    int64_t arena_size = 0;
+    std::map<int64_t, int64_t> free_list;  // offset -> size, coalesced
+    // Grow-only FP64 staging for seeded split-form slots (the FP64 intensities
+    // the state is packed from).
+    double* staging = nullptr;
+    int64_t staging_bytes = 0;
 };
 
 namespace {
 
-void free_slot(Slot& s) {
-    hipFree(s.I[0]);
-    hipFree(s.I[1]);
-    hipFree(s.state[0]);
-    hipFree(s.state[1]);
-    hipFree(s.cell_inv);
-    hipFree(s.B);
+constexpr int64_t kArenaAlign = 256;
+
+hipError_t dev_alloc(rsh_batch b, size_t bytes, void** p) {
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    if (!b->arena) return hipMalloc(p, bytes);
+    const int64_t need = ((int64_t)bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    for (auto it = b->free_list.begin(); it != b->free_list.end(); ++it) {
+        if (it->second < need) continue;
+        const int64_t off = it->first, size = it->second;
+        b->free_list.erase(it);
+        if (size > need) b->free_list[off + need] = size - need;
+        *p = b->arena + off;
+        return hipSuccess;
+    }
+    return hipErrorOutOfMemory;
+}
+
+void dev_free(rsh_batch b, void* p, size_t bytes) {
+    if (!p) return;
+    char* c = static_cast<char*>(p);
+    if (!b->arena || c < b->arena || c >= b->arena + b->arena_size) {
+        hipFree(p);
+        return;
+    }
+    int64_t off = c - b->arena;
+    int64_t size = ((int64_t)bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    auto next = b->free_list.lower_bound(off);
+    if (next != b->free_list.end() && off + size == next->first) {  // merge with the following block
+        size += next->second;
+        next = b->free_list.erase(next);
+    }
+    if (next != b->free_list.begin()) {  // merge with the preceding block
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += size;
+            return;
+        }
+    }
+    b->free_list[off] = size;
+}
+
+void free_slot(rsh_batch b, Slot& s) {
+    dev_free(b, s.I[0], s.fbytes);
+    dev_free(b, s.I[1], s.fbytes);
+    dev_free(b, s.state[0], s.sbytes);
+    dev_free(b, s.state[1], s.sbytes);
+    dev_free(b, s.cell_inv, s.cbytes);
+    dev_free(b, s.B, s.fbytes);
+    s.I[0] = s.I[1] = nullptr;
+    s.state[0] = s.state[1] = nullptr;
+    s.cell_inv = nullptr;
+    s.B = nullptr;
 }
 
 int slot_rb(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.rb_per_pass : s.splan.rb_per_pass; }
@@ -242,7 +301,9 @@ int rsh_batch_destroy(rsh_batch b) {
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
     destroy_graph(b);
-    for (auto& s : b->slots) free_slot(s);
+    for (auto& s : b->slots) free_slot(b, s);
+    hipFree(b->arena);
+    hipFree(b->staging);
     for (auto& kv : b->classes) {
         hipFree(kv.second.d_items);
         hipFree(kv.second.d_n_items);
@@ -255,9 +316,17 @@ int rsh_batch_destroy(rsh_batch b) {
     return RSH_OK;
 }
 
-int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B, double albedo,
-                  int32_t form, int32_t* slot) {
-    if (!b || !ds || !slot || n_cells <= 0 || !d_I0 || !d_B) return fail(RSH_E_ARG, "invalid batch_add arguments");
+}  // extern "C"
+
+namespace {
+
+// Adds a slot whose initial state and source term are either copied from the
+// caller's device arrays (d_I0, d_B) or generated on the device from a seed
+// (I0 ~ log-normal(0, 1) with seed 2 s + 1, B ~ U(0.5, 2) with seed 2 s + 2).
+int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int form, const double* d_I0,
+             const double* d_B, const uint64_t* seed, int32_t* slot) {
+    if (!b || !ds || !slot || n_cells <= 0 || (!seed && (!d_I0 || !d_B)))
+        return fail(RSH_E_ARG, "invalid batch_add arguments");
     if (footprint(form, 1, 1) < 0) return fail(RSH_E_ARG, "unknown form");
     if (ds->device != b->device) return fail(RSH_E_ARG, "direction set lives on another device");
     int id = -1;
@@ -268,10 +337,10 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
         }
     if (id < 0) return fail(RSH_E_NOMEM, "batch is full");
     RSH_HIP_TRY(hipSetDevice(b->device));
-    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
-    // The batch stream is non-blocking: order its copies of d_I0 / d_B after all
-    // work already submitted to the null stream (where callers usually made them).
-    {
+    if (!seed) {
+        // The batch stream is non-blocking: order its copies of d_I0 / d_B after
+        // all work already submitted to the null stream (where callers usually
+        // made them).
         hipEvent_t ready;
         RSH_HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
         hipError_t er = hipEventRecord(ready, nullptr);
@@ -289,40 +358,67 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
     s.splan = plan_split(ds->m, ds->m);
     const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
     const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
-    hipError_t e = hipMalloc(&s.B, fbytes);
+    s.fbytes = fbytes;
+    s.sbytes = sbytes;
+    s.cbytes = form == RSH_FORM_FP64 ? 0 : (size_t)n_cells * sizeof(float);
+    hipError_t e = dev_alloc(b, fbytes, (void**)&s.B);
     if (e == hipSuccess && form == RSH_FORM_FP64) {
-        if ((e = hipMalloc(&s.I[0], fbytes)) == hipSuccess) e = hipMalloc(&s.I[1], fbytes);
+        if ((e = dev_alloc(b, fbytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, fbytes, (void**)&s.I[1]);
     } else if (e == hipSuccess) {
-        if ((e = hipMalloc(&s.state[0], sbytes)) == hipSuccess && (e = hipMalloc(&s.state[1], sbytes)) == hipSuccess)
-            e = hipMalloc(&s.cell_inv, (size_t)n_cells * sizeof(float));
+        if ((e = dev_alloc(b, sbytes, &s.state[0])) == hipSuccess && (e = dev_alloc(b, sbytes, &s.state[1])) == hipSuccess)
+            e = dev_alloc(b, s.cbytes, (void**)&s.cell_inv);
     }
     if (e != hipSuccess) {
-        free_slot(s);
+        free_slot(b, s);
+        if (e == hipErrorOutOfMemory && b->arena) return fail(RSH_E_NOMEM, "batch arena has no free block this large");
         return hip_fail(e, "batch slot allocation");
     }
     s.bytes = footprint(form, n_cells, ds->m);
     const int p = b->parity;  // the next iteration reads buffer p
-    if ((e = hipMemcpyAsync(s.B, d_B, fbytes, hipMemcpyDeviceToDevice, b->stream)) == hipSuccess) {
-        if (form == RSH_FORM_FP64) {
-            e = hipMemcpyAsync(s.I[p], d_I0, fbytes, hipMemcpyDeviceToDevice, b->stream);
-        } else {
-            // padding positions of the other buffer are never written by the epilogue
-            if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess &&
-                (e = launch_cell_scale(d_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
-                e = launch_pack_state(form, d_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+    const double* src_I0 = d_I0;
+    if (seed) {
+        const uint64_t sd = *seed;
+        e = launch_fill_uniform(s.B, (int64_t)ds->m * n_cells, 0.5, 2.0, sd * 2 + 2, b->stream);
+        if (e == hipSuccess && form == RSH_FORM_FP64) {
+            e = launch_fill_lognormal(s.I[p], (int64_t)ds->m * n_cells, 0.0, 1.0, sd * 2 + 1, b->stream);
+        } else if (e == hipSuccess) {
+            if ((int64_t)fbytes > b->staging_bytes) {
+                // grow-only; the old staging may still be read by queued packs
+                RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+                hipFree(b->staging);
+                b->staging = nullptr;
+                b->staging_bytes = 0;
+                if ((e = hipMalloc(&b->staging, fbytes)) == hipSuccess) b->staging_bytes = (int64_t)fbytes;
+            }
+            if (e == hipSuccess)
+                e = launch_fill_lognormal(b->staging, (int64_t)ds->m * n_cells, 0.0, 1.0, sd * 2 + 1, b->stream);
+            src_I0 = b->staging;
         }
+    } else {
+        if ((e = hipMemcpyAsync(s.B, d_B, fbytes, hipMemcpyDeviceToDevice, b->stream)) == hipSuccess &&
+            form == RSH_FORM_FP64)
+            e = hipMemcpyAsync(s.I[p], d_I0, fbytes, hipMemcpyDeviceToDevice, b->stream);
+    }
+    if (e == hipSuccess && form != RSH_FORM_FP64) {
+        // padding positions of the other buffer are never written by the epilogue
+        if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess &&
+            (e = launch_cell_scale(src_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
+            e = launch_pack_state(form, src_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
     }
     if (e != hipSuccess) {
-        free_slot(s);
+        free_slot(b, s);
         return hip_fail(e, "batch slot initialisation");
     }
     const PanelEntry* pe = nullptr;
     int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe);
     if (rc) {
-        free_slot(s);
+        free_slot(b, s);
         return rc;
     }
-    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    // Copies from caller arrays complete before returning (the caller may free
+    // them); seeded slots need no wait: everything above is ordered on the
+    // batch stream before the next iteration.
+    if (!seed) RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     s.active = true;
     BatchProblem& P = b->h_probs[id];
     P = BatchProblem{};
@@ -351,12 +447,44 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
     return RSH_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I0, const double* d_B, double albedo,
+                  int32_t form, int32_t* slot) {
+    return add_impl(b, ds, n_cells, albedo, form, d_I0, d_B, nullptr, slot);
+}
+
+int rsh_batch_add_seeded(rsh_batch b, rsh_dirset ds, int64_t n_cells, uint64_t seed, double albedo, int32_t form,
+                         int32_t* slot) {
+    return add_impl(b, ds, n_cells, albedo, form, nullptr, nullptr, &seed, slot);
+}
+
+int rsh_batch_reserve(rsh_batch b, int64_t bytes) {
+    if (!b || bytes < 0) return fail(RSH_E_ARG, "invalid reserve arguments");
+    for (const auto& s : b->slots)
+        if (s.active) return fail(RSH_E_STATE, "rsh_batch_reserve needs an empty batch");
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    RSH_HIP_TRY(hipStreamSynchronize(b->stream));
+    hipFree(b->arena);
+    b->arena = nullptr;
+    b->arena_size = 0;
+    b->free_list.clear();
+    if (bytes == 0) return RSH_OK;
+    const int64_t size = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    RSH_HIP_TRY(hipMalloc(&b->arena, (size_t)size));
+    b->arena_size = size;
+    b->free_list[0] = size;
+    return RSH_OK;
+}
+
 int rsh_batch_remove(rsh_batch b, int32_t slot) {
     if (!b || slot < 0 || slot >= b->capacity || !b->slots[slot].active) return fail(RSH_E_ARG, "invalid slot");
     RSH_HIP_TRY(hipSetDevice(b->device));
     RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     Slot& s = b->slots[slot];
-    free_slot(s);
+    free_slot(b, s);
     s = Slot{};
     BatchProblem& P = b->h_probs[slot];
     unsigned long long* r = P.resid;

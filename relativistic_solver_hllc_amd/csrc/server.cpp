@@ -5,7 +5,8 @@
 // tolerance. The server keeps a FIFO of pending requests and one batch of
 // resident ones (rsh_batch). Each step:
 //   1. admits pending requests, in arrival order, while a slot is free and the
-//      HBM footprint of the admitted set stays within the budget; their
+//      HBM footprint of the admitted set stays within the budget (reserved once
+//      as the batch's device arena, so admission allocates nothing); their
 //      intensities and source terms are generated on the device from the seed;
 //   2. runs `iters_per_step` iterations of every resident request (grouped
 //      launches replayed from a captured hipGraph);
@@ -89,32 +90,16 @@ int get_dirset(rsh_server srv, int m, int phase, double g, rsh_dirset* out) {
     return RSH_OK;
 }
 
+// Admits a pending request into a batch slot; its data are generated on the
+// device in the slot. RSH_E_NOMEM = no arena block fits right now (the request
+// stays pending).
 int admit(rsh_server srv, int id) {
     Request& r = srv->reqs[id];
     rsh_dirset ds = nullptr;
     int rc = get_dirset(srv, r.desc.n_dirs, r.desc.phase, r.desc.g, &ds);
     if (rc) return rc;
-    const int64_t count = (int64_t)r.desc.n_dirs * r.desc.n_cells;
-    double *I0 = nullptr, *B = nullptr;
-    void* st = nullptr;
-    rsh_batch_stream(srv->batch, &st);
-    hipError_t e;
-    if ((e = hipMalloc(&I0, count * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(&B, count * sizeof(double))) != hipSuccess) {
-        hipFree(I0);
-        return hip_fail(e, "request staging");
-    }
-    const uint64_t seed = r.desc.seed;
-    if ((e = launch_fill_lognormal(I0, count, 0.0, 1.0, seed * 2 + 1, (hipStream_t)st)) != hipSuccess ||
-        (e = launch_fill_uniform(B, count, 0.5, 2.0, seed * 2 + 2, (hipStream_t)st)) != hipSuccess) {
-        hipFree(I0);
-        hipFree(B);
-        return hip_fail(e, "request data");
-    }
     int slot = -1;
-    rc = rsh_batch_add(srv->batch, ds, r.desc.n_cells, I0, B, r.desc.albedo, r.desc.form, &slot);
-    hipFree(I0);
-    hipFree(B);
+    rc = rsh_batch_add_seeded(srv->batch, ds, r.desc.n_cells, r.desc.seed, r.desc.albedo, r.desc.form, &slot);
     if (rc) return rc;
     r.slot = slot;
     r.status = RSH_REQ_RUNNING;
@@ -161,6 +146,13 @@ int rsh_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_slot
     srv->slot_req.assign(max_slots, -1);
     int rc = rsh_batch_create(device, max_slots, 1, &srv->batch);
     if (rc) return rc;
+    // The budget is reserved once as the slots' arena: admission then costs no
+    // device allocation and no device-wide synchronisation.
+    rc = rsh_batch_reserve(srv->batch, hbm_budget_bytes);
+    if (rc) {
+        rsh_batch_destroy(srv->batch);
+        return rc;
+    }
     now_s();
     *out = srv.release();
     return RSH_OK;
@@ -230,9 +222,10 @@ int rsh_server_step(rsh_server srv, int32_t* running, int32_t* finished_now) {
         const Request& r = srv->reqs[id];
         const bool slot_free = std::find(srv->slot_req.begin(), srv->slot_req.end(), -1) != srv->slot_req.end();
         if (!slot_free || srv->used + r.bytes > srv->budget) break;
-        srv->pending.pop_front();
         int rc = admit(srv, id);
+        if (rc == RSH_E_NOMEM) break;  // arena fragmented: retry after the next retirements
         if (rc) return rc;
+        srv->pending.pop_front();
     }
     int nrun = 0;
     for (int s : srv->slot_req)

@@ -240,6 +240,17 @@ class Batch:
                C.byref(slot))
         return slot.value
 
+    def add_seeded(self, ds: DirectionSet, n_cells: int, seed: int, albedo: float, form: str = "fp64") -> int:
+        """Slot with I0 ~ log-normal(0,1) and B ~ U(0.5,2) generated on the device from seed."""
+        slot = C.c_int32()
+        N.call("rsh_batch_add_seeded", self.handle, ds.handle, n_cells, int(seed) & (2**64 - 1), float(albedo),
+               FORMS[form], C.byref(slot))
+        return slot.value
+
+    def reserve(self, nbytes: int) -> None:
+        """One device arena for the slot buffers (batch must be empty)."""
+        N.call("rsh_batch_reserve", self.handle, int(nbytes))
+
     def remove(self, slot: int) -> None:
         N.call("rsh_batch_remove", self.handle, slot)
 

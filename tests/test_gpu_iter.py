@@ -214,3 +214,66 @@ def test_add_is_ordered_after_null_stream_work(gpu, form):
         assert rel(got, ref) < (1e-12 if form == "fp64" else 2e-6)
         for a in (I0, B, out):
             a.free()
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("arena", [False, True])
+def test_seeded_slot_equals_copied_slot(gpu, form, arena):
+    """rsh_batch_add_seeded generates in the slot exactly what rsh_fill_*_dev with
+    the same seeds produce; arena-carved slots compute the same bits."""
+    m, n, seed = 200, 3000, 1234
+    xyz, w, F = rs.fibonacci_directions(m)
+    with rs.DirectionSet(xyz, w, F, phase="hg", g=0.6, normalize=True) as ds, rs.Batch(4) as b:
+        if arena:
+            b.reserve(64 << 20)
+        I0, B = D.DeviceArray((m, n)), D.DeviceArray((m, n))
+        rs.fill_lognormal_dev(I0, 0.0, 1.0, seed=seed * 2 + 1)
+        rs.fill_uniform_dev(B, 0.5, 2.0, seed=seed * 2 + 2)
+        s1 = b.add(ds, n, I0, B, 0.5, form=form)
+        s2 = b.add_seeded(ds, n, seed, 0.5, form=form)
+        b.run(5)
+        r, it = b.residuals()
+        out1, out2 = D.DeviceArray((m, n)), D.DeviceArray((m, n))
+        b.state(s1, out1)
+        b.state(s2, out2)
+        assert np.array_equal(out1.to_host(), out2.to_host())
+        assert r[s1] == r[s2] and it[s1] == it[s2] == 5
+        for a in (I0, B, out1, out2):
+            a.free()
+
+
+def test_arena_reuse_and_exhaustion(gpu):
+    """Slots carved from a reserved arena are freed and coalesced on removal;
+    a problem larger than any free block is refused with RSH_E_NOMEM and fits
+    again once its neighbours are gone."""
+    m = 200
+    xyz, w, F = rs.fibonacci_directions(m)
+    per = rs.Batch.footprint("fp64", 1000, m)
+    with rs.DirectionSet(xyz, w, F, normalize=True) as ds, rs.Batch(8) as b:
+        b.reserve(3 * per + 3 * 256 * 3)
+        a1 = b.add_seeded(ds, 1000, 1, 0.5)
+        a2 = b.add_seeded(ds, 1000, 2, 0.5)
+        a3 = b.add_seeded(ds, 1000, 3, 0.5)
+        with pytest.raises(RuntimeError, match="RSH_E_NOMEM"):
+            b.add_seeded(ds, 1000, 4, 0.5)
+        b.run(2)
+        b.remove(a1)
+        b.remove(a2)
+        with pytest.raises(RuntimeError, match="RSH_E_NOMEM"):
+            b.add_seeded(ds, 2500, 5, 0.5)  # more than the two freed blocks together
+        big = b.add_seeded(ds, 2000, 5, 0.5)  # fits the coalesced block of a1 + a2
+        b.run(3)
+        r, it = b.residuals()
+        assert it[big] == 3 and it[a3] == 5
+        # the problem in the reused memory matches the same problem alone
+        out = D.DeviceArray((m, 2000))
+        b.state(big, out)
+        got = out.to_host()
+        out.free()
+    with rs.DirectionSet(xyz, w, F, normalize=True) as ds, rs.Batch(2) as b2:
+        s = b2.add_seeded(ds, 2000, 5, 0.5)
+        b2.run(3)
+        out = D.DeviceArray((m, 2000))
+        b2.state(s, out)
+        assert np.array_equal(out.to_host(), got)
+        out.free()
