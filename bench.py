@@ -199,12 +199,20 @@ def moments_section(D, rs, ds, illum, xyz, w, F, n, m, steps, stream, cells):
         worst = max(worst, float(np.max(err / (d if k.startswith("div") else e))))
     for a in [*geo, *outs.values()]:
         a.free()
+    traffic = None
+    pmc_path = os.path.join(REPO, "profiles", "pmc_moments_f64_1572864x400.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+        if pmc.get("n_cells") == n and pmc.get("n_dirs") == m:
+            traffic = pmc.get("hbm_bytes_per_launch")
     log(f"[bench] moments: {t:.3f} ms, {byts / (t * 1e-3) / 1e9:.0f} GB/s, scaled max err {worst:.2e}")
     return {"workload": "radiation moments (energy, stream, impuls, divstream, divimpuls) of the same "
                         "1,572,864-cell x 4-face intensities, M=400, synthetic cell geometry",
             "kernel": "moments_f64 (FP64 VALU, HBM-bound)", "ms": t, "cells_per_s": n / (t * 1e-3),
             "roofline": {"bound": "hbm", "achieved": byts / (t * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": byts},
+                         "frac": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                         "algorithmic_bytes_per_launch": byts},
             "parity": {"sampled_cells": int(len(cells)), "max_scaled_err_vs_oracle": worst, "tolerance": 1e-12,
                        "scale": "per cell: max_f sum_j w_j |I_jf| / F (x sum_f area_f / volume for divergences)"}}
 
