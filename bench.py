@@ -124,7 +124,7 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
         e1.record(stream)
         pack_ms = e0.elapsed_ms(e1)
         entry = {"state": "cell-level (face average packed once)", "state_bytes": sb, "pack_ms": pack_ms,
-                 "kernel": "resident-panel" if form != "bf16x3" else "streaming-panel (8 waves)"}
+                 "kernel": "resident-panel"}
         for label, o, ob in (("fp64_out", out, 8), ("fp32_out", out32, 4)):
             t = timed(lambda: ds.scatter_split(form, state, scale, 1, n, 0, m, o, out_f32=(ob == 4), stream=stream))
             byts = sb + n * m * ob

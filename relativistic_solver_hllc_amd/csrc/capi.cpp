@@ -199,15 +199,12 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     if (waves == 1 && !resident_ok)
         return fail(RSH_E_UNSUPPORTED, "resident-panel kernel needs cell-level state (faces = 1) and a panel slice "
                                        "that fits LDS with at most one row group per CU of an XCD");
-    // Auto: the resident-panel kernel for two-part forms when its row groups
-    // keep at least 3/4 of the CUs busy (M = 800 needs 17 groups of 3 row
-    // blocks: 17 of 32 CUs per XCD). For bf16x3 the streaming kernel measured
-    // faster (4.2 vs 4.7 ms at 1.57 M cells x 400 directions).
+    // Auto: the resident-panel kernel when its row groups keep at least 3/4 of
+    // the CUs busy (M = 800 needs 17 groups of 3 row blocks: 17 of 32 CUs per
+    // XCD), else the streaming kernel with 8 waves.
     const int per_xcd = device_cus(ds->device) / 8;
     if (waves == 0)
-        waves = (resident_ok && split_terms(form) == 2 && 4 * sets * rplan.passes >= 3 * per_xcd)
-                    ? 1
-                    : split_default_waves();
+        waves = (resident_ok && 4 * sets * rplan.passes >= 3 * per_xcd) ? 1 : split_default_waves();
     const PanelEntry* pe = nullptr;
     if (waves == 1) {
         int rc = get_panel(ds, k0, k1, faces, form, st, &pe, &rplan);

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     typedef typename Fmt<FMT>::V V;
     constexpr int XD = kResXd;
     const int steps = STEPS > 0 ? STEPS : steps_rt;
-    extern __shared__ v4u lds[];  // [steps][RBW][NT][64]
+    extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then the row group's 16 RBW row scales
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -64,6 +64,12 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     const v4u* __restrict__ src = panel + (int64_t)rg * units;
     for (int64_t u = threadIdx.x; u < units; u += 64 * kResWaves) lds[u] = src[u];
     const int row0 = rg * RBW * 16;
+    // Row scales once per workgroup into LDS: scale loads from global memory in
+    // every epilogue measurably broke the row groups' L2 sharing of the state
+    // (2.4x the state bytes fetched from HBM).
+    float* rscale = reinterpret_cast<float*>(lds + units);
+    for (int r = threadIdx.x; r < RBW * 16; r += 64 * kResWaves)
+        rscale[r] = row0 + r < rows ? row_inv[row0 + r] : 0.0f;
     __syncthreads();
 
     // 2. chunks of this XCD, round-robin over its sets
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
                     const int row = row0 + i * 16 + 4 * g + r;
                     if (row < rows) {
                         const double v =
-                            ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)row_inv[row] * cinv;
+                            ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)rscale[row - row0] * cinv;
                         if (out_f32)
                             __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
                         else
@@ -202,7 +208,7 @@ bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int*
     const int nt = split_terms(form);
     const int steps = (m + 31) / 32;
     const int rb_need = (rows + 15) / 16;
-    const int per_rb = steps * nt * 1024;
+    const int per_rb = steps * nt * 1024 + 16 * 4;  // panel fragments + row scales
     int rbw_max = kResLdsBytes / per_rb;
     if (rbw_max > res_max_rbw(nt)) rbw_max = res_max_rbw(nt);
     if (rbw_max < 1 || rb_need < 1) return false;
@@ -223,7 +229,7 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
                                  void* d_out, int64_t ld, int out_f32, hipStream_t st) {
     if (n <= 0 || rows <= 0) return hipSuccess;
     const int nt = split_terms(form), fmt = split_fmt(form);
-    const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * nt * 1024;
+    const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * nt * 1024 + (size_t)pl.rb_per_pass * 16 * 4;
     if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > res_max_rbw(nt) || sets < 1) return hipErrorInvalidValue;
     const int64_t tiles = (n + 15) / 16;
     const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
