@@ -195,10 +195,11 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     RSH_HIP_TRY(hipSetDevice(ds->device));
     SplitPlan rplan{};
     int sets = 0;
-    const bool resident_ok = faces == 1 && plan_split_resident(form, ds->m, k1 - k0, device_cus(ds->device), &rplan, &sets);
+    const bool resident_ok = faces == 1 && ld < kSplitResidentMaxLd &&
+                             plan_split_resident(form, ds->m, k1 - k0, device_cus(ds->device), &rplan, &sets);
     if (waves == 1 && !resident_ok)
-        return fail(RSH_E_UNSUPPORTED, "resident-panel kernel needs cell-level state (faces = 1) and a panel slice "
-                                       "that fits LDS with at most one row group per CU of an XCD");
+        return fail(RSH_E_UNSUPPORTED, "resident-panel kernel needs cell-level state (faces = 1), ld_out below 2^24 "
+                                       "and a panel slice that fits LDS with at most one row group per CU of an XCD");
     // Auto: the resident-panel kernel when its row groups keep at least 3/4 of
     // the CUs busy (M = 800 needs 17 groups of 3 row blocks: 17 of 32 CUs per
     // XCD), else the streaming kernel with 8 waves.

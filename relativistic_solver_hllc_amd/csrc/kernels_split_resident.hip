@@ -19,10 +19,10 @@
 //
 // Per wave: two 16-cell tiles x RBW row blocks, so every A fragment read from
 // LDS feeds two MFMAs per product; MFMAs are issued product-major so that no
-// MFMA depends on the one before it; state vectors are prefetched two K-steps
-// ahead with ordinary (L2-allocating) loads, since the other row groups of the
-// XCD read the same vectors. Products and their order per (row block, tile) are
-// those of the streaming kernel (main hi*hi; cross hi*lo + lo*hi, and for three
+// MFMA depends on the one before it; state vectors are prefetched several
+// K-steps ahead, across chunk boundaries too, with ordinary (L2-allocating)
+// loads, since the other row groups of the XCD read the same vectors. Products
+// and their order per (row block, tile) are those of the streaming kernel (main hi*hi; cross hi*lo + lo*hi, and for three
 // parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi), so the kernels give
 // bit-identical results on the same state and panel.
 #include "mfma_split_core.hpp"
@@ -33,10 +33,30 @@ namespace {
 
 constexpr int kResWaves = 8;   // waves per workgroup (2 per SIMD)
 constexpr int kResTiles = 2;   // 16-cell tiles per wave
-constexpr int kResXd = 2;      // state prefetch depth (K-steps ahead)
 constexpr int kResChunkTiles = kResWaves * kResTiles;  // 256 cells per chunk
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kResLdsBytes = 160 * 1024;
+// Buffer descriptor word 3 for raw (untyped) gfx9 buffer access: 32-bit data
+// format, so that the hardware does not treat the descriptor as invalid.
+constexpr int kBufWord3 = 0x00020000;
+// Cache policy of the result stores: non-temporal (each result is written once
+// and not read again by this kernel).
+constexpr int kNontemporal = 2;
+
+// State prefetch depth in K-steps (registers: XD x tiles x parts x 4 VGPRs).
+// Measured at 1.57 M cells x 400 directions: two parts 2 / 4 / 6 steps ->
+// 2.21 / 2.15 / 1.99 ms with FP64 output; three parts 2 / 3 / 4 -> 3.29 /
+// 3.25 / 3.35 ms.
+template <int NT>
+constexpr int res_xd() { return NT == 2 ? 6 : 3; }
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
 
 // STEPS > 0: the K-step count is a compile-time constant and the K loop is
 // fully unrolled, so the state prefetch crosses no loop back-edge (where the
@@ -47,7 +67,7 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     const v4u* __restrict__ panel, const float* __restrict__ row_inv, void* __restrict__ out, int64_t ld, int rows,
     int out_f32, int groups, int sets, int64_t chunks) {
     typedef typename Fmt<FMT>::V V;
-    constexpr int XD = kResXd;
+    constexpr int XD = res_xd<NT>();
     const int steps = STEPS > 0 ? STEPS : steps_rt;
     extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then the row group's 16 RBW row scales
     const int lane = threadIdx.x & 63;
@@ -75,22 +95,61 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     // 2. chunks of this XCD, round-robin over its sets
     const int64_t c_begin = chunks * xcd / kXcds, c_end = chunks * (xcd + 1) / kXcds;
     const int64_t tiles = (n + 15) / 16;
-    for (int64_t chunk = c_begin + set; chunk < c_end; chunk += sets) {
-        const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
-        // Tiles past the end read the last tile (in bounds) and are not written:
-        // unconditional loads keep the prefetch visible to the wait counters.
-        const v4u* xsrc[kResTiles];
+    const int64_t tile_units = (int64_t)steps * NT * 64;  // 16-byte vectors per 16-cell tile
+    const int tile_bytes = (int)(tile_units * 16);
+
+    // A chunk's state is read through one buffer descriptor: the uniform base and
+    // the K-step offsets live in scalar registers, so a tile needs one VGPR
+    // offset instead of a 64-bit address per K-step. Tiles past the end of the
+    // state re-read the chunk's last tile (in range) and are not written.
+    __amdgpu_buffer_rsrc_t xs;
+    int xoff[kResTiles];
+    auto bind_chunk = [&](int64_t chunk) {
+        const int64_t ct_chunk = chunk * kResChunkTiles;
+        const int ntile = (int)(tiles - ct_chunk < kResChunkTiles ? tiles - ct_chunk : kResChunkTiles);
+        xs = __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(state + ct_chunk * tile_units), (short)0,
+                                               ntile * tile_bytes, kBufWord3);
+#pragma unroll
+        for (int q = 0; q < kResTiles; q++) {
+            const int tile = wave * kResTiles + q;
+            xoff[q] = (tile < ntile ? tile : ntile - 1) * tile_bytes + lane * 16;
+        }
+    };
+    v4u xr[XD][kResTiles][NT];
+    auto load_x = [&](auto slot_tag, int t) {
+        constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
         for (int q = 0; q < kResTiles; q++)
-            xsrc[q] = state + (ct0 + q < tiles ? ct0 + q : tiles - 1) * (int64_t)steps * NT * 64 + lane;
-        v4u xr[XD][kResTiles][NT];
-        auto load_x = [&](auto slot_tag, int t) {
-            constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++)
+            for (int p = 0; p < NT; p++)
+                xr[sl][q][p] = __builtin_bit_cast(
+                    v4u, __builtin_amdgcn_raw_buffer_load_b128(xs, xoff[q] + p * 1024, t * NT * 1024, 0));
+    };
+    // First XD K-steps of a chunk. For every chunk after the first they are
+    // issued before the previous chunk's result stores: loads and stores retire
+    // through one in-order counter, so a load issued after the stores could
+    // only be waited for together with all of them.
+    auto prefetch_head = [&]() {
+        static_for<0, XD>([&](auto u) {
+            if (u.value < steps) load_x(u, u.value);
+        });
+    };
+
+    int64_t chunk = c_begin + set;
+    if (chunk < c_end) {
+        bind_chunk(chunk);
+        prefetch_head();
+    }
+    for (; chunk < c_end; chunk += sets) {
+        const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
+        // Cell scales now, ahead of the next chunk's prefetch: a load issued
+        // after those could only be waited for together with all of them.
+        float cinv[kResTiles];
 #pragma unroll
-                for (int p = 0; p < NT; p++) xr[sl][q][p] = xsrc[q][((int64_t)t * NT + p) * 64];
-        };
+        for (int q = 0; q < kResTiles; q++) {
+            const int64_t cell = (ct0 + q) * 16 + (lane & 15);
+            cinv[q] = cell_inv[cell < n ? cell : n - 1];
+        }
         v4f accm[RBW][kResTiles], accx[RBW][kResTiles];
 #pragma unroll
         for (int i = 0; i < RBW; i++)
@@ -142,40 +201,79 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
                 round(accx, 0);
             }
         };
-        load_x(std::integral_constant<int, 0>{}, 0);
-        if (steps > 1) load_x(std::integral_constant<int, 1>{}, 1);
         if constexpr (STEPS > 0) {
-#pragma unroll
-            for (int t = 0; t < STEPS; t += 2) {
-                kstep(std::integral_constant<int, 0>{}, t);
-                if (t + 1 < STEPS) kstep(std::integral_constant<int, 1>{}, t + 1);
-            }
+            static_for<0, STEPS>([&](auto t) { kstep(std::integral_constant<int, t.value % XD>{}, t.value); });
         } else {
-            for (int t = 0; t < steps; t += 2) {
-                kstep(std::integral_constant<int, 0>{}, t);
-                if (t + 1 < steps) kstep(std::integral_constant<int, 1>{}, t + 1);
-            }
+            for (int t = 0; t < steps; t += XD)
+                static_for<0, XD>([&](auto u) {
+                    if (t + u.value < steps) kstep(u, t + u.value);
+                });
         }
-        // epilogue: FP64 combination of main and cross terms, exact scale removal
+        __builtin_amdgcn_sched_barrier(0);
+        if (chunk + sets < c_end) {
+            bind_chunk(chunk + sets);
+            prefetch_head();
+        }
+        __builtin_amdgcn_sched_barrier(0);
+
+        // Epilogue: FP64 combination of main and cross terms, exact scale removal.
+        // Row scales are re-read from LDS per chunk: an opaque zero offset keeps
+        // the compiler from hoisting all 4 RBW of them out of the chunk loop,
+        // where they would be spilled. Stores go through one buffer descriptor
+        // per row block (uniform base, row offset r * ld in a scalar register,
+        // one VGPR offset per tile), again to keep 64-bit addresses out of the
+        // vector registers.
+        int zoff;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
+        const float* __restrict__ rs = rscale + 4 * g + zoff;
+        v4f rsv[RBW];  // this lane's 4 row scales of each row block, read together
 #pragma unroll
-        for (int q = 0; q < kResTiles; q++) {
-            const int64_t cell = (ct0 + q) * 16 + (lane & 15);
-            if (ct0 + q >= tiles || cell >= n) continue;
-            const double cinv = (double)cell_inv[cell];
+        for (int i = 0; i < RBW; i++) rsv[i] = *reinterpret_cast<const v4f*>(rs + i * 16);
+        double cd[kResTiles];
 #pragma unroll
-            for (int i = 0; i < RBW; i++)
+        for (int q = 0; q < kResTiles; q++) cd[q] = (double)cinv[q];
+        auto store_all = [&](auto elem_tag, auto full_tag) {
+            typedef decltype(elem_tag) OT;
+            constexpr bool kFull = decltype(full_tag)::value;
+            const int esz = (int)sizeof(OT);
+            int voff[kResTiles];
+            bool ok[kResTiles];
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) {
+                const int64_t cell = (ct0 + q) * 16 + (lane & 15);
+                ok[q] = cell < n;
+                voff[q] = (int)((4 * g * ld + cell) * esz);
+            }
+#pragma unroll
+            for (int i = 0; i < RBW; i++) {
+                const int rb_row = row0 + i * 16;
+                if (!kFull && rb_row >= rows) break;
+                const __amdgpu_buffer_rsrc_t os = __builtin_amdgcn_make_buffer_rsrc(
+                    static_cast<OT*>(out) + (int64_t)rb_row * ld, (short)0, (int)(16 * ld * esz), kBufWord3);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int row = row0 + i * 16 + 4 * g + r;
-                    if (row < rows) {
-                        const double v =
-                            ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)rscale[row - row0] * cinv;
-                        if (out_f32)
-                            __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
+                    const double rsc = (double)rsv[i][r];
+#pragma unroll
+                    for (int q = 0; q < kResTiles; q++) {
+                        if (!ok[q] || (!kFull && rb_row + 4 * g + r >= rows)) continue;
+                        const double v = ((double)accm[i][q][r] + (double)accx[i][q][r]) * rsc * cd[q];
+                        if constexpr (sizeof(OT) == 4)
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), os,
+                                                                  voff[q], (int)(r * ld * esz), kNontemporal);
                         else
-                            __builtin_nontemporal_store(v, static_cast<double*>(out) + (int64_t)row * ld + cell);
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), os, voff[q],
+                                                                  (int)(r * ld * esz), kNontemporal);
                     }
                 }
+            }
+        };
+        const bool full = row0 + RBW * 16 <= rows;
+        if (out_f32) {
+            if (full) store_all(0.0f, std::true_type{});
+            else store_all(0.0f, std::false_type{});
+        } else {
+            if (full) store_all(0.0, std::true_type{});
+            else store_all(0.0, std::false_type{});
         }
     }
 }
@@ -230,7 +328,8 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
     if (n <= 0 || rows <= 0) return hipSuccess;
     const int nt = split_terms(form), fmt = split_fmt(form);
     const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * nt * 1024 + (size_t)pl.rb_per_pass * 16 * 4;
-    if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > res_max_rbw(nt) || sets < 1) return hipErrorInvalidValue;
+    if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > res_max_rbw(nt) || sets < 1 || ld >= kSplitResidentMaxLd)
+        return hipErrorInvalidValue;
     const int64_t tiles = (n + 15) / 16;
     const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
     const int blocks = kXcds * sets * pl.passes;

@@ -75,7 +75,10 @@ hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t
                              void* d_state, hipStream_t st);
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
                               const SplitPlan& pl, void* d_panel, float* d_row_inv, hipStream_t st);
-// Resident-panel kernel (cell-level state only): plan and launch.
+// Resident-panel kernel (cell-level state only): plan and launch. Its result
+// stores address 16 output rows through one 32-bit buffer range, so the output
+// leading dimension must stay below kSplitResidentMaxLd cells.
+constexpr int64_t kSplitResidentMaxLd = int64_t(1) << 24;
 bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets);
 hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const float* d_cell_inv,
                                  const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int sets, int rows,
