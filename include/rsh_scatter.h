@@ -427,6 +427,19 @@ int rsh_moments(rsh_problem p, int32_t stream, double* energy, double* stream_ou
 int rsh_problem_moment_buffers(rsh_problem p, double** d_energy, double** d_stream, double** d_impuls,
                                double** d_divstream, double** d_divimpuls);
 
+/* Solution dump in simple-bin files named main_dir + name (plain string
+ * concatenation, as the reference does): Illum.bin = face average of the first
+ * direction [N]; after rsh_moments also energy.bin [N], stream.bin [N] Vector3,
+ * impuls.bin [N] 3x3, and with geometry divstream.bin [N], divimpuls.bin [N]
+ * Vector3 (count = N, elements of 8, 24 or 72 bytes). Waits for all work on the
+ * device first. Replaces the ILLUM part of WriteFileSolution
+ * (file_module/writer_bin.cpp:15-44); its CUDA branch writes only N doubles of
+ * stream / impuls / divimpuls (WRITE_FILE with n = grid.size), here the whole
+ * arrays are written with the element layout of its CPU branch
+ * (WRITE_FILE_VECTOR), which CompareFiles (utils/rebuild_solve.cpp:601-669)
+ * reads as Vector3 / Matrix3. */
+int rsh_write_solution(rsh_problem p, const char* main_dir);
+
 /* Milliseconds of the last rsh_scatter kernel on each stream id (hipEvents).
  * Replaces CudaEventPrint (cuda/cuda_solve.cu:16-36), whose events the
  * reference never creates. ms must hold RSH_STREAM_COUNT values; streams

@@ -16,6 +16,7 @@ _EXPORTS = {
     "partition": "scatter",
     "read_directions": "scatter",
     "read_simple_bin": "scatter",
+    "compare_files": "scatter",
     "selftest_mfma_f64": "scatter",
     "write_directions": "scatter",
     "write_simple_bin": "scatter",

@@ -149,6 +149,7 @@ PROTOTYPES = {
     "rsh_problem_set_geometry": (C.c_int, [vp, vp, vp, vp]),
     "rsh_moments": (C.c_int, [vp, i32, vp, vp, vp, vp, vp]),
     "rsh_problem_moment_buffers": (C.c_int, [vp, P_vp, P_vp, P_vp, P_vp, P_vp]),
+    "rsh_write_solution": (C.c_int, [vp, cp]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

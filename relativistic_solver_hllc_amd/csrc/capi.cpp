@@ -17,6 +17,7 @@
 #include <string>
 #include <tuple>
 #include <vector>
+#include <vector>
 
 #include "rsh_objects.hpp"
 
@@ -840,11 +841,44 @@ int rsh_moments(rsh_problem p, int32_t stream, double* energy, double* stream_ou
                          geo ? p->d_geo + 4 * nf : nullptr, dev[0], dev[1], dev[2], geo ? dev[3] : nullptr,
                          geo ? dev[4] : nullptr, 0, st);
     if (rc) return rc;
+    p->mom_valid = true;
+    p->div_valid = geo;
     double* host[5] = {energy, stream_out, impuls, divstream, divimpuls};
     for (int i = 0; i < 5; i++)
         if (host[i] && n > 0)
             RSH_HIP_TRY(hipMemcpyAsync(host[i], dev[i], (size_t)kMomWidth[i] * n * sizeof(double),
                                        hipMemcpyDeviceToHost, st));
+    return RSH_OK;
+}
+
+int rsh_write_solution(rsh_problem p, const char* main_dir) {
+    if (!p || !main_dir) return fail(RSH_E_ARG, "null problem or directory");
+    const int64_t n = p->n, nf = n * p->faces;
+    if (n > INT32_MAX) return fail(RSH_E_ARG, "the solution format's count is a 32-bit int");
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    RSH_HIP_TRY(hipDeviceSynchronize());
+    const std::string dir(main_dir);
+    // Illum.bin: face average of the first direction, summed in face order
+    // then divided (GetDirectionIllumFromFace(size, 0, ...)).
+    std::vector<double> faces((size_t)std::max<int64_t>(nf, 1)), cell((size_t)std::max<int64_t>(n, 1));
+    if (nf > 0) RSH_HIP_TRY(hipMemcpy(faces.data(), p->d_illum, (size_t)nf * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < n; c++) {
+        double acc = 0.0;
+        for (int f = 0; f < p->faces; f++) acc += faces[(size_t)(c * p->faces + f)];
+        cell[(size_t)c] = acc / p->faces;
+    }
+    int rc = rsh_write_simple_bin((dir + "Illum.bin").c_str(), 8, cell.data(), n);
+    if (rc || !p->mom_valid) return rc;
+    static const char* names[5] = {"energy.bin", "stream.bin", "impuls.bin", "divstream.bin", "divimpuls.bin"};
+    std::vector<double> host((size_t)std::max<int64_t>(kMomPerCell * n, 1));
+    if (n > 0)
+        RSH_HIP_TRY(hipMemcpy(host.data(), p->d_mom, (size_t)(kMomPerCell * n) * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 5; i++) {
+        if (i >= 3 && !p->div_valid) break;
+        rc = rsh_write_simple_bin((dir + names[i]).c_str(), 8 * kMomWidth[i], host.data() + kMomOff[i] * n, n);
+        if (rc) return rc;
+    }
     return RSH_OK;
 }
 

@@ -57,6 +57,7 @@ struct rsh_problem_s {
     double* d_S = nullptr;
     double* d_geo = nullptr;  // normals [n][faces][3], areas [n][faces], volume [n]
     double* d_mom = nullptr;  // energy [n], stream [n][3], impuls [n][9], divstream [n], divimpuls [n][3]
+    bool mom_valid = false, div_valid = false;  // what the last rsh_moments computed
     hipStream_t streams[RSH_STREAM_COUNT] = {};
     hipEvent_t upload_ev[RSH_STREAM_COUNT] = {};
     bool upload_pending[RSH_STREAM_COUNT] = {};

@@ -97,6 +97,34 @@ def write_simple_bin(path: str, data, width: int = 1) -> None:
     N.call("rsh_write_simple_bin", path.encode(), esize, a.ctypes.data if a.size else None, a.size // width)
 
 
+# Element widths (doubles) of the solution files, as CompareFiles reads them
+# (double, Vector3, Matrix3; utils/rebuild_solve.cpp:601-669).
+SOLUTION_FILES = {"Illum.bin": 1, "energy.bin": 1, "stream.bin": 3, "impuls.bin": 9, "divstream.bin": 1,
+                  "divimpuls.bin": 3}
+COMPARE_WIDTH = {"double": 1, "Vector3": 3, "Matrix3": 9}
+
+
+def compare_files(file1: str, file2: str, type_: str) -> Tuple[int, float, float]:
+    """CompareFiles (utils/rebuild_solve.cpp:579-669) for double / Vector3 / Matrix3 files:
+    (index of the largest difference, that difference, sum of differences); differences are
+    |a - b| for doubles and the Euclidean (Frobenius) norm of a - b for vectors and matrices.
+    The first file's count drives the loop, as in the reference."""
+    if type_ not in COMPARE_WIDTH:
+        raise ValueError(f"unknown type {type_!r}: double, Vector3 or Matrix3")
+    wd = COMPARE_WIDTH[type_]
+    a = read_simple_bin(file1, np.float64, wd).reshape(-1, wd)
+    b = read_simple_bin(file2, np.float64, wd).reshape(-1, wd)
+    if len(b) < len(a):
+        raise ValueError(f"{file2} holds {len(b)} elements, {file1} {len(a)}")
+    d = np.abs(a[:, 0] - b[: len(a), 0]) if wd == 1 else np.sqrt(((a - b[: len(a)]) ** 2).sum(axis=1))
+    best, idx, total = -1.0, -1, 0.0
+    for i, x in enumerate(d.tolist()):  # sequential sum and first maximum, as the reference loop
+        total += x
+        if best < x:
+            best, idx = x, i
+    return idx, best, total
+
+
 def partition(np_: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
     send = np.empty(np_, dtype=np.int32)
     disp = np.empty(np_, dtype=np.int32)
@@ -422,6 +450,12 @@ class Problem:
                opt(divimpuls))
         if sync:
             N.call("rsh_sync", self.handle, stream)
+
+    def write_solution(self, main_dir: str) -> None:
+        """Solution dump (WriteFileSolution, file_module/writer_bin.cpp:15-44): Illum.bin and, after
+        calculate_all_param, energy / stream / impuls (/ divstream / divimpuls with geometry) .bin files
+        named main_dir + name."""
+        N.call("rsh_write_solution", self.handle, main_dir.encode())
 
     def moment_buffers(self) -> Tuple[int, int, int, int, int]:
         ptrs = [C.c_void_p() for _ in range(5)]

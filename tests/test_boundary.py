@@ -116,3 +116,27 @@ def test_argument_errors_without_device():
 
 def test_device_count_is_callable():
     assert rs.device_count() >= 0
+
+
+def test_compare_files_matches_definition(tmp_path):
+    """CompareFiles (utils/rebuild_solve.cpp:579-669) on simple-bin files written
+    through the library: first index of the largest difference and the
+    sequential sum, |a-b| for doubles and the Euclidean norm for Vector3 /
+    Matrix3 elements; the first file's count drives the loop."""
+    rng = np.random.default_rng(5)
+    for type_, wd in (("double", 1), ("Vector3", 3), ("Matrix3", 9)):
+        a = rng.integers(-8, 8, size=(40, wd)).astype(np.float64)  # exact differences
+        b = a.copy()
+        b[7] += 0.5           # largest difference, planted twice: the first index wins
+        b[31] += 0.5
+        b[12] -= 0.25
+        f1, f2 = str(tmp_path / f"a{wd}.bin"), str(tmp_path / f"b{wd}.bin")
+        rs.write_simple_bin(f1, a, wd)
+        rs.write_simple_bin(f2, np.vstack([b, rng.normal(size=(3, wd))]), wd)  # longer second file
+        idx, worst, total = rs.compare_files(f1, f2, type_)
+        d = [abs(x[0] - y[0]) if wd == 1 else float(np.sqrt(sum((xi - yi) ** 2 for xi, yi in zip(x, y))))
+             for x, y in zip(a.tolist(), b.tolist())]
+        assert idx == 7 and worst == max(d)
+        assert total == pytest.approx(sum(d), rel=1e-15)
+    with pytest.raises(ValueError):
+        rs.compare_files(f1, f2, "int64")

@@ -213,3 +213,43 @@ def test_moments_config2_full_size(gpu):
     finally:
         for a in [illum, *geo, *outs.values(), *outs2.values()]:
             a.free()
+
+
+def test_write_solution_files(gpu, tmp_path):
+    """Solution dump after CalculateAllParam: Illum.bin is the first direction's
+    face average (bit-exact: faces summed in order, then / 4), the moment files
+    hold N elements of 1, 3 or 9 doubles and match the oracle; without geometry
+    the divergence files are not written; CompareFiles of a file with itself is 0."""
+    rng = np.random.default_rng(44)
+    n, m = 2500, 30
+    xyz, w, F = rs.fibonacci_directions(m)
+    normals, areas, volume, _ = random_tetra_cells(n, rng)
+    illum = rng.lognormal(size=(m, n, 4))
+    ref = orc.moments(illum, xyz, w, F, normals, areas, volume, threads=16)
+    expect_illum = (((illum[0, :, 0] + illum[0, :, 1]) + illum[0, :, 2]) + illum[0, :, 3]) / 4
+    for with_geo in (True, False):
+        d = tmp_path / ("geo" if with_geo else "nogeo")
+        d.mkdir()
+        with rs.Problem(n, xyz, w, F, 0, m) as p:
+            if with_geo:
+                p.set_geometry(normals, areas, volume)
+            p.copy_illum_on_device(np.ascontiguousarray(illum), m * n * 4)
+            p.write_solution(str(d) + "/")  # before any moments: Illum.bin only
+            assert sorted(os.listdir(d)) == ["Illum.bin"]
+            p.calculate_all_param(rs.STREAM_PARAMS)
+            p.write_solution(str(d) + "/")
+        names = sorted(os.listdir(d))
+        want = sorted(k for k in rs.SOLUTION_FILES if with_geo or not k.startswith("div"))
+        assert names == want
+        got_illum = rs.read_simple_bin(str(d / "Illum.bin"))
+        assert got_illum.shape == (n,) and np.array_equal(got_illum, expect_illum)
+        got = {}
+        for k in WIDTH:
+            f = d / f"{k}.bin"
+            if f.exists():
+                a = rs.read_simple_bin(str(f), np.float64, rs.SOLUTION_FILES[f"{k}.bin"])
+                assert a.shape[0] == n
+                got[k] = a if WIDTH[k] > 1 else a.reshape(n)
+        check(got, ref, illum, w, F, areas, volume)
+        idx, worst, total = rs.compare_files(str(d / "stream.bin"), str(d / "stream.bin"), "Vector3")
+        assert (worst, total) == (0.0, 0.0) and idx == 0
