@@ -446,6 +446,73 @@ int rsh_write_solution(rsh_problem p, const char* main_dir);
  * without a timed launch report -1. */
 int rsh_event_times(rsh_problem p, float* ms);
 
+/* ------------------------------------------------------------------ sweep
+ * Transport sweep by short characteristics on a tetrahedral mesh: the step
+ * that produces the face intensities [M][N][4] the scattering integral reads.
+ * Face f of a cell is the face opposite its vertex f; pairs[c*4+f] is the
+ * neighbour's face id (cell*4 + face) or a boundary type. */
+
+/* Boundary types of eBoundaryTypes (global_def.h:45-51); BoundaryConditions
+ * (solve_module/illum/illum_utils.cpp:407-471) gives 1 on OUT_SOURCE and 0 on
+ * the others. */
+enum { RSH_BOUND_FREE = -1, RSH_BOUND_LOCK = -2, RSH_BOUND_OUT_SOURCE = -3, RSH_BOUND_INNER_SOURCE = -4 };
+
+/* Unit cube [0,1]^3 in n_edge^3 cubes of 6 tetrahedra (N = 6 n_edge^3 cells,
+ * conforming Kuhn split): vertices [N][4][3], pairs [N][4] (faces on x = 0 are
+ * OUT_SOURCE, other boundary faces FREE). Synthetic mesh with no reference
+ * counterpart (the reference reads VTK meshes); n_edge <= 100. */
+int rsh_mesh_cube_tetra(int32_t n_edge, double* vertices, int32_t* pairs);
+
+/* Cell geometry from vertices [N][4][3]; each output may be NULL: outward unit
+ * normals [N][4][3] and areas [N][4] (normals.bin / squares.bin layouts),
+ * volume [N], and the three nodes of each face [N][4][3][3] (X.bin layout,
+ * BasePointTetra, global_def.h:92-99; node j at weight 2/3 on face vertex j). */
+int rsh_tetra_geometry(int64_t n, const double* vertices, double* normals, double* areas, double* volume,
+                       double* nodes);
+
+/* Per-direction sweep structures for unit direction omega, as the reference's
+ * make_illum_struct writes them (make_illum_struct/
+ * struct_short_characteristics_main.cpp:128-165): face_state [N] (bit f set =
+ * in face, n_f . omega < -1e-10, GetInAndOutFaces), order [N] (an upwind
+ * cell order, graph<k>.bin) and, for every node of every out face in that
+ * order, the distance s back along -omega to the cell's in-face plane and that
+ * face's id (vec_x0 / LocX0<k>.bin records). cap = capacity of x0_s / x0_face;
+ * *n_records receives the count needed (RSH_E_NOMEM if above cap). */
+int rsh_sweep_prepare_direction(int64_t n, const double* vertices, const int32_t* pairs, const double* omega,
+                                int32_t* order, int32_t* face_state, double* x0_s, uint8_t* x0_face, int64_t cap,
+                                int64_t* n_records);
+
+typedef struct rsh_sweep_s* rsh_sweep;
+
+/* Sweep object for n_cells cells and n_dirs directions on a device: pairs
+ * [N][4] (checked: symmetric neighbours or boundary types), nodes [N][4][3][3]
+ * (needed only by class 0; may be NULL). Replaces the pairs / X.bin part of
+ * ReadIllumGeometry (file_module/reader_bin.cpp:164-203). */
+int rsh_sweep_create(int32_t device, int64_t n_cells, int32_t n_dirs, const int32_t* pairs, const double* nodes,
+                     rsh_sweep* out);
+int rsh_sweep_destroy(rsh_sweep s);
+
+/* Direction k's structures in the reference's formats (sorted_id_cell,
+ * face_states, vec_x0's s and in_face_id in traversal order), checked (order a
+ * permutation that is upwind, every record naming an in face, record count =
+ * 3 x out faces) and uploaded; the cell levels for the parallel sweep are
+ * derived here. Replaces the per-direction part of ReadIllumGeometry. */
+int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const int32_t* face_state,
+                            const double* x0_s, const uint8_t* x0_face, int64_t n_records);
+/* Number of dependency levels of direction k (cells of one level run in parallel). */
+int rsh_sweep_levels(rsh_sweep s, int32_t k, int32_t* levels);
+
+/* One sweep over all directions on a stream (NULL = null stream), device
+ * arrays: scattering [M][ld] (grid.scattering), new face intensities
+ * d_illum_new [M][N][4]; with d_illum_old (distinct, [M][N][4]) also
+ * d_norm [M] = ReCalcIllum's max |old - new| / new per direction.
+ * cls = solve_mode.class_vtk: 0, 1, 2, 3 or 10 (GetCurIllum,
+ * Illum_async.cpp:259-430; 11 couples to the gas-dynamics state and is not
+ * offered). Replaces the direction loop of MPI_CalculateIllumAsync
+ * (solve_module/illum/Illum_async.cpp:725-784) and ReCalcIllum (:472-510). */
+int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t ld, const double* d_illum_old,
+                  double* d_illum_new, double* d_norm, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,7 @@ _PROTOS = {
     "orc_max_threads": (C.c_int, []),
     "orc_moments": (None, [vp, i64, i32, i32, vp, vp, f64, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
     "orc_read_simple_bin": (i64, [C.c_char_p, i32, vp, i64]),
+    "orc_sweep": (f64, [i64, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32]),
 }
 _cache = {}
 
@@ -184,3 +185,26 @@ def read_simple_bin(path, dtype=np.float64):
     if lib.orc_read_simple_bin(path.encode(), dt.itemsize, out.ctypes.data, n) != n:
         raise IOError(f"orc_read_simple_bin({path}) failed")
     return out
+
+
+def sweep(per_dir, pairs, scattering, illum_old, cls=1, nodes=None, threads=0, variant="parity"):
+    """Sequential transport sweep (oracle/sweep_oracle.cpp, Illum_async.cpp:725-784 + ReCalcIllum).
+    per_dir: list over directions of (order, face_state, s, in_face); scattering [M][N];
+    illum_old [M][N][4]. Returns (illum_new, norm); norm = -2 flags malformed input."""
+    lib = load(variant)
+    M = len(per_dir)
+    pairs = _c(pairs, np.int32)
+    n = pairs.shape[0]
+    order = _c(np.stack([d[0] for d in per_dir]), np.int32)
+    state = _c(np.stack([d[1] for d in per_dir]), np.int32)
+    counts = [len(d[2]) for d in per_dir]
+    begin = _c(np.concatenate([[0], np.cumsum(counts)]), np.int64)
+    s = _c(np.concatenate([d[2] for d in per_dir]) if sum(counts) else np.zeros(1))
+    f = _c(np.concatenate([d[3] for d in per_dir]) if sum(counts) else np.zeros(1, np.uint8), np.uint8)
+    S = _c(scattering)
+    illum = _c(illum_old).copy()
+    nd = None if nodes is None else _c(nodes)
+    norm = lib.orc_sweep(n, M, order.ctypes.data, state.ctypes.data, s.ctypes.data, f.ctypes.data, begin.ctypes.data,
+                         pairs.ctypes.data, None if nd is None else nd.ctypes.data, S.ctypes.data, cls,
+                         illum.ctypes.data, threads)
+    return illum, norm

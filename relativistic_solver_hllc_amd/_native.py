@@ -150,6 +150,14 @@ PROTOTYPES = {
     "rsh_moments": (C.c_int, [vp, i32, vp, vp, vp, vp, vp]),
     "rsh_problem_moment_buffers": (C.c_int, [vp, P_vp, P_vp, P_vp, P_vp, P_vp]),
     "rsh_write_solution": (C.c_int, [vp, cp]),
+    "rsh_mesh_cube_tetra": (C.c_int, [i32, vp, vp]),
+    "rsh_tetra_geometry": (C.c_int, [i64, vp, vp, vp, vp, vp]),
+    "rsh_sweep_prepare_direction": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(i64)]),
+    "rsh_sweep_create": (C.c_int, [i32, i64, i32, vp, vp, P_vp]),
+    "rsh_sweep_destroy": (C.c_int, [vp]),
+    "rsh_sweep_set_direction": (C.c_int, [vp, i32, vp, vp, vp, vp, i64]),
+    "rsh_sweep_levels": (C.c_int, [vp, i32, P_i32]),
+    "rsh_sweep_run": (C.c_int, [vp, i32, vp, i64, vp, vp, vp, vp]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

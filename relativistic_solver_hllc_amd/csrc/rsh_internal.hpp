@@ -75,6 +75,21 @@ hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t
                              void* d_state, hipStream_t st);
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
                               const SplitPlan& pl, void* d_panel, float* d_row_inv, hipStream_t st);
+// Transport sweep: one direction's level-ordered cells and vec_x0 records on
+// the device (see kernels_sweep.hip).
+struct SweepDirDev {
+    const int32_t* cells;      // [n] cells sorted by level (stable within a level)
+    const int32_t* level_ptr;  // [levels + 1] offsets into cells
+    const int64_t* x0_off;     // [n] first record of each cell (indexed by cell)
+    const uint8_t* state;      // [n] face-state bits (bit f set = in face)
+    const double* s;           // [records] distance back to the in face
+    const uint8_t* in_face;    // [records] in-face id
+    int32_t levels;
+    int32_t pad;
+};
+hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, const int32_t* d_pairs,
+                        const double* d_nodes, const double* d_S, int64_t ld, const double* d_old, double* d_new,
+                        double* d_norm, hipStream_t st);
 // Resident-panel kernel (cell-level state only): plan and launch. Its result
 // stores address 16 output rows through one 32-bit buffer range, so the output
 // leading dimension must stay below kSplitResidentMaxLd cells.

@@ -1,0 +1,226 @@
+// Host side of the transport sweep: validates the reference-format sweep
+// structures of each direction (sorted_id_cell, face states, vec_x0 records),
+// derives the level of every cell, uploads everything once and launches the
+// sweep kernel. Replaces ReadIllumGeometry's in-memory result
+// (file_module/reader_bin.cpp:164-203) and the direction loop of
+// MPI_CalculateIllumAsync (solve_module/illum/Illum_async.cpp:725-784).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rsh_internal.hpp"
+#include "rsh_scatter.h"
+
+using namespace rsh;
+
+struct rsh_sweep_s {
+    int device = 0;
+    int64_t n = 0;
+    int m = 0;
+    int32_t* d_pairs = nullptr;
+    double* d_nodes = nullptr;
+    std::vector<int32_t> pairs;  // host copy for validation and levels
+    struct Dir {
+        void* block = nullptr;  // one device allocation per direction
+        SweepDirDev dev{};
+        int64_t records = 0;
+        bool set = false;
+    };
+    std::vector<Dir> dirs;
+    SweepDirDev* d_table = nullptr;
+    bool table_dirty = true;
+    std::mutex mu;
+};
+
+namespace {
+
+bool valid_bound(int32_t t) { return t >= RSH_BOUND_INNER_SOURCE && t <= RSH_BOUND_FREE; }
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+int rsh_sweep_create(int32_t device, int64_t n_cells, int32_t n_dirs, const int32_t* pairs, const double* nodes,
+                     rsh_sweep* out) {
+    if (!out || !pairs || n_cells <= 0 || n_dirs <= 0 || n_cells > INT32_MAX / 4)
+        return fail(RSH_E_ARG, "invalid sweep size or null pointer");
+    for (int64_t i = 0; i < n_cells * 4; i++) {
+        const int32_t p = pairs[i];
+        if (p >= 0 ? (p >= n_cells * 4 || p / 4 == i / 4 || pairs[p] != i) : !valid_bound(p))
+            return fail(RSH_E_ARG, "pairs: entry " + std::to_string(i) +
+                                       " is neither a symmetric neighbour face nor a boundary type -1..-4");
+    }
+    auto* s = new rsh_sweep_s();
+    s->device = device;
+    s->n = n_cells;
+    s->m = n_dirs;
+    s->pairs.assign(pairs, pairs + n_cells * 4);
+    s->dirs.resize((size_t)n_dirs);
+    auto cleanup = [&](int rc) {
+        rsh_sweep_destroy(s);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(fail(RSH_E_HIP, "hipSetDevice failed"));
+    if (hipMalloc(&s->d_pairs, (size_t)n_cells * 4 * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&s->d_table, (size_t)n_dirs * sizeof(SweepDirDev)) != hipSuccess)
+        return cleanup(fail(RSH_E_NOMEM, "sweep: device allocation failed"));
+    if (hipMemcpy(s->d_pairs, pairs, (size_t)n_cells * 4 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(RSH_E_HIP, "sweep: pairs upload failed"));
+    if (nodes) {
+        const size_t b = (size_t)n_cells * 36 * sizeof(double);
+        if (hipMalloc(&s->d_nodes, b) != hipSuccess) return cleanup(fail(RSH_E_NOMEM, "sweep: node allocation failed"));
+        if (hipMemcpy(s->d_nodes, nodes, b, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(RSH_E_HIP, "sweep: node upload failed"));
+    }
+    *out = s;
+    return RSH_OK;
+}
+
+int rsh_sweep_destroy(rsh_sweep s) {
+    if (!s) return RSH_OK;
+    hipSetDevice(s->device);
+    for (auto& d : s->dirs)
+        if (d.block) hipFree(d.block);
+    if (s->d_pairs) hipFree(s->d_pairs);
+    if (s->d_nodes) hipFree(s->d_nodes);
+    if (s->d_table) hipFree(s->d_table);
+    delete s;
+    return RSH_OK;
+}
+
+int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const int32_t* face_state,
+                            const double* x0_s, const uint8_t* x0_face, int64_t n_records) {
+    if (!s || k < 0 || k >= s->m || !order || !face_state || n_records < 0 || (n_records > 0 && (!x0_s || !x0_face)))
+        return fail(RSH_E_ARG, "invalid direction, null pointer or record count");
+    const int64_t n = s->n;
+    std::vector<int32_t> pos((size_t)n, -1), level((size_t)n, 0);
+    for (int64_t h = 0; h < n; h++) {
+        const int32_t c = order[h];
+        if (c < 0 || c >= n || pos[(size_t)c] >= 0) return fail(RSH_E_ARG, "order is not a permutation of the cells");
+        pos[(size_t)c] = (int32_t)h;
+    }
+    std::vector<int64_t> x0_off((size_t)n);
+    int64_t r = 0;
+    int32_t levels = 0;
+    for (int64_t h = 0; h < n; h++) {
+        const int32_t c = order[h];
+        const int st = face_state[c];
+        if (st < 0 || st > 15) return fail(RSH_E_ARG, "face state outside 0..15");
+        x0_off[(size_t)c] = r;
+        int32_t lv = 0;
+        for (int f = 0; f < 4; f++) {
+            const int32_t p = s->pairs[(size_t)(c * 4 + f)];
+            const bool in = (st >> f) & 1;
+            if (!in) {
+                for (int j = 0; j < 3; j++, r++) {
+                    if (r >= n_records) return fail(RSH_E_ARG, "fewer x0 records than out-face nodes");
+                    const int g = x0_face[r];
+                    if (g > 3 || !((st >> g) & 1))
+                        return fail(RSH_E_ARG, "x0 record " + std::to_string(r) + " does not name an in face");
+                }
+            }
+            if (p < 0) continue;
+            const int32_t u = p / 4;
+            // Reads across an in face, and a face both cells treat as out
+            // (written by both: the later one in the order wins), order u first.
+            const bool dep = in || (!((face_state[u] >> (p % 4)) & 1) && pos[(size_t)u] < h);
+            if (!dep) continue;
+            if (pos[(size_t)u] > h)
+                return fail(RSH_E_ARG, "order is not upwind: cell " + std::to_string(c) + " reads cell " +
+                                           std::to_string(u) + " that comes later");
+            lv = std::max(lv, level[(size_t)u] + 1);
+        }
+        level[(size_t)c] = lv;
+        levels = std::max(levels, lv + 1);
+    }
+    if (r != n_records) return fail(RSH_E_ARG, "more x0 records than out-face nodes");
+    // Cells by level, stable in the given order (counting sort).
+    std::vector<int32_t> level_ptr((size_t)levels + 1, 0), cells((size_t)n);
+    for (int64_t c = 0; c < n; c++) level_ptr[(size_t)level[(size_t)c] + 1]++;
+    for (int32_t l = 0; l < levels; l++) level_ptr[(size_t)l + 1] += level_ptr[(size_t)l];
+    std::vector<int32_t> fill(level_ptr.begin(), level_ptr.end() - 1);
+    for (int64_t h = 0; h < n; h++) {
+        const int32_t c = order[h];
+        cells[(size_t)fill[(size_t)level[(size_t)c]]++] = c;
+    }
+    std::vector<uint8_t> st8((size_t)n);
+    for (int64_t c = 0; c < n; c++) st8[(size_t)c] = (uint8_t)face_state[c];
+
+    const size_t b_cells = align256((size_t)n * 4), b_lp = align256(((size_t)levels + 1) * 4),
+                 b_off = align256((size_t)n * 8), b_st = align256((size_t)n), b_s = align256((size_t)n_records * 8),
+                 b_f = align256((size_t)std::max<int64_t>(n_records, 1));
+    std::lock_guard<std::mutex> lock(s->mu);
+    RSH_HIP_TRY(hipSetDevice(s->device));
+    auto& d = s->dirs[(size_t)k];
+    if (d.block) RSH_HIP_TRY(hipFree(d.block));
+    d.block = nullptr;
+    d.set = false;
+    if (hipMalloc(&d.block, b_cells + b_lp + b_off + b_st + b_s + b_f) != hipSuccess)
+        return fail(RSH_E_NOMEM, "sweep: direction allocation failed");
+    char* base = static_cast<char*>(d.block);
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        return bytes ? hipMemcpy(base + off, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    size_t off = 0;
+    RSH_HIP_TRY(put(off, cells.data(), (size_t)n * 4));
+    d.dev.cells = reinterpret_cast<const int32_t*>(base + off);
+    off += b_cells;
+    RSH_HIP_TRY(put(off, level_ptr.data(), ((size_t)levels + 1) * 4));
+    d.dev.level_ptr = reinterpret_cast<const int32_t*>(base + off);
+    off += b_lp;
+    RSH_HIP_TRY(put(off, x0_off.data(), (size_t)n * 8));
+    d.dev.x0_off = reinterpret_cast<const int64_t*>(base + off);
+    off += b_off;
+    RSH_HIP_TRY(put(off, st8.data(), (size_t)n));
+    d.dev.state = reinterpret_cast<const uint8_t*>(base + off);
+    off += b_st;
+    RSH_HIP_TRY(put(off, x0_s, (size_t)n_records * 8));
+    d.dev.s = reinterpret_cast<const double*>(base + off);
+    off += b_s;
+    RSH_HIP_TRY(put(off, x0_face, (size_t)n_records));
+    d.dev.in_face = reinterpret_cast<const uint8_t*>(base + off);
+    d.dev.levels = levels;
+    d.records = n_records;
+    d.set = true;
+    s->table_dirty = true;
+    return RSH_OK;
+}
+
+int rsh_sweep_levels(rsh_sweep s, int32_t k, int32_t* levels) {
+    if (!s || k < 0 || k >= s->m || !levels) return fail(RSH_E_ARG, "invalid direction or null pointer");
+    if (!s->dirs[(size_t)k].set) return fail(RSH_E_STATE, "direction not set");
+    *levels = s->dirs[(size_t)k].dev.levels;
+    return RSH_OK;
+}
+
+int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t ld, const double* d_illum_old,
+                  double* d_illum_new, double* d_norm, void* stream) {
+    if (!s || !d_scattering || !d_illum_new || ld < s->n) return fail(RSH_E_ARG, "null pointer or ld < n_cells");
+    if (cls != 0 && cls != 1 && cls != 2 && cls != 3 && cls != 10)
+        return fail(RSH_E_UNSUPPORTED, "class_vtk must be 0, 1, 2, 3 or 10 (11 couples to the gas dynamics state)");
+    if (cls == 0 && !s->d_nodes) return fail(RSH_E_ARG, "class 0 needs the face nodes (rsh_sweep_create nodes)");
+    if (d_illum_old && (d_illum_old == d_illum_new || !d_norm))
+        return fail(RSH_E_ARG, "the residual needs distinct old and new arrays and a norm array");
+    std::lock_guard<std::mutex> lock(s->mu);
+    for (const auto& d : s->dirs)
+        if (!d.set) return fail(RSH_E_STATE, "every direction needs rsh_sweep_set_direction first");
+    RSH_HIP_TRY(hipSetDevice(s->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (s->table_dirty) {
+        std::vector<SweepDirDev> table;
+        for (const auto& d : s->dirs) table.push_back(d.dev);
+        RSH_HIP_TRY(hipMemcpy(s->d_table, table.data(), table.size() * sizeof(SweepDirDev), hipMemcpyHostToDevice));
+        s->table_dirty = false;
+    }
+    RSH_HIP_TRY(launch_sweep(cls, s->d_table, s->m, s->n, s->d_pairs, s->d_nodes, d_scattering, ld, d_illum_old,
+                             d_illum_new, d_norm, st));
+    return RSH_OK;
+}
+
+}  // extern "C"
