@@ -217,6 +217,96 @@ def moments_section(D, rs, ds, illum, xyz, w, F, n, m, steps, stream, cells):
                        "scale": "per cell: max_f sum_j w_j |I_jf| / F (x sum_f area_f / volume for divergences)"}}
 
 
+def sweep_section(D, rs, edge=32, m=400, steps=5, cpu=True):
+    """Transport sweep (short characteristics, next row after the moments): a 6 edge^3-cell
+    tetrahedral cube, M Fibonacci directions, the test-task class (Q = 1, alpha = 1, beta =
+    1/2), synthetic log-normal scattering; one launch sweeps every direction and computes
+    ReCalcIllum's residual. Also one full transport iteration (sweep + FP64 scattering)."""
+    from oracle import orc
+    from relativistic_solver_hllc_amd import sweep as sw
+
+    v, pairs = sw.mesh_cube_tetra(edge)
+    n = v.shape[0]
+    xyz, w, F = rs.fibonacci_directions(m)
+    t0 = time.perf_counter()
+    per = sw.prepare(v, pairs, xyz, threads=16)
+    prep_s = time.perf_counter() - t0
+    sp = sw.Sweep(n, m, pairs)
+    t0 = time.perf_counter()
+    sp.set_all(per)
+    upload_s = time.perf_counter() - t0
+    levels = np.array([sp.levels(k) for k in range(m)])
+    rng = np.random.default_rng(0x5EED + 7)
+    S = rng.lognormal(size=(m, n))
+    old = np.ones((m, n, 4))
+    d_S, d_old = D.DeviceArray.from_host(S), D.DeviceArray.from_host(old)
+    d_new, d_norm = D.DeviceArray((m, n, 4)), D.DeviceArray((m,))
+    stream = D.Stream(0)
+    sp.run(1, d_S, d_new, d_old, d_norm, stream=stream)
+    stream.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        sp.run(1, d_S, d_new, d_old, d_norm, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    t = e0.elapsed_ms(e1) / steps
+    got = d_new.to_host()
+    norm = float(d_norm.to_host().max())
+    # parity: the sequential oracle on a slice of directions
+    sub = list(range(0, m, max(1, m // 16)))[:16]
+    ref, ref_norm = orc.sweep([per[k] for k in sub], pairs, S[sub], old[sub], cls=1, threads=16)
+    err = float(np.max(np.abs(got[sub] - ref) / np.maximum(np.abs(ref), 1e-300)))
+    # full transport iteration: sweep, then the FP64 scattering integral of the new intensities
+    ds = rs.DirectionSet(xyz, w, F)
+    d_Sit = D.DeviceArray((m, n))
+    ds.scatter_f64(d_new, 4, n, 0, m, d_Sit, stream=stream)
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        sp.run(1, d_Sit, d_new, d_old, d_norm, stream=stream)
+        ds.scatter_f64(d_new, 4, n, 0, m, d_Sit, stream=stream)
+    e1.record(stream)
+    stream.synchronize()
+    t_iter = e0.elapsed_ms(e1) / steps
+    # algorithmic bytes: per (cell, direction) the level list entry, record offset, face state and
+    # S value; 9 bytes per record; one FP64 read per interior in face; 4 face values written; the
+    # residual reads 4 new and 4 old values
+    records = sum(len(d[2]) for d in per)
+    in_interior = sum(int(np.sum(((d[1][:, None] >> np.arange(4)) & 1).astype(bool) & (pairs >= 0))) for d in per)
+    byts = m * n * (4 + 8 + 1 + 8) + records * 9 + in_interior * 8 + m * n * 4 * 8 * 3
+    cpu_res = None
+    if cpu:
+        threads = min(16, os.cpu_count() or 1)
+        c0 = time.perf_counter()
+        _, _ = orc.sweep(per, pairs, S, old, cls=1, threads=threads, variant="o3")
+        c_s = time.perf_counter() - c0
+        cpu_res = {"value": n * m / c_s, "unit": "cell-direction updates/s", "cores": threads, "kind": "port",
+                   "seconds": c_s, "sample": f"the whole sweep ({m} directions x {n} cells), oracle -O3, "
+                                             "OpenMP over directions as the reference"}
+    for a in (d_S, d_old, d_new, d_norm, d_Sit):
+        a.free()
+    ds.close()
+    sp.close()
+    gbs = byts / (t * 1e-3) / 1e9
+    log(f"[bench] sweep {n} cells x {m} dirs: {t:.3f} ms ({n * m / (t * 1e-3):.3e} cell-dir/s, {gbs:.0f} GB/s), "
+        f"levels {levels.min()}-{levels.max()}, max rel err vs oracle {err:.2e}; iteration {t_iter:.3f} ms"
+        + (f"; cpu {cpu_res['seconds']:.2f} s" if cpu_res else ""))
+    return {"workload": f"transport sweep: unit cube in {edge}^3 x 6 = {n} tetrahedra, M={m} Fibonacci directions, "
+                        "class_vtk 1 (test task), log-normal scattering, ReCalcIllum residual",
+            "kernel": "sweep_kernel<1> (one workgroup per direction, level-parallel)", "ms": t,
+            "cell_directions_per_s": n * m / (t * 1e-3), "levels_min": int(levels.min()),
+            "levels_mean": float(levels.mean()), "levels_max": int(levels.max()),
+            "prepare_s": prep_s, "upload_s": upload_s, "records": records,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": byts},
+            "iteration_ms": t_iter, "iteration": "sweep + FP64 scattering integral (mfma_f64) of the new faces",
+            "residual": norm,
+            "parity": {"directions_checked": len(sub), "max_rel_err_vs_oracle": err, "tolerance": 1e-12,
+                       "oracle_residual": ref_norm},
+            "cpu_baseline": cpu_res}
+
+
 def config4_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
     """Uneven problems: N log-uniform in [n_lo, n_hi] rounded to a structured cube 6 n^3,
     M in {200, 400, 800}, phase Thomson or HG g in {0.3, 0.6, 0.9}."""
@@ -337,6 +427,7 @@ def main() -> int:
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision (config 3) section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems (config 4) section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) section")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the transport-sweep section")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="before the benchmark, run the plain copy kernel 3x over this many GB "
                          "(reference for FETCH_SIZE / WRITE_SIZE counter scaling)")
@@ -426,6 +517,10 @@ def main() -> int:
         cpu = cpu_baseline(illum, xyz, w, F, n, m, threads, k_slice, "o3")
         log(f"[bench] cpu baseline: {cpu['value']:.3e} pairs/s on {threads} threads")
 
+    sweep = None
+    if not args.no_sweep:
+        sweep = sweep_section(D, rs, steps=max(args.steps // 4, 2), cpu=not args.no_cpu)
+
     batch = None
     serve = None
     if not args.no_batch or not args.no_serve:
@@ -474,6 +569,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "moments": moments,
         "split": split,
+        "sweep": sweep,
         "batch": batch,
         "serve": serve,
     }

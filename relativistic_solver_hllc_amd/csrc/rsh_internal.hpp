@@ -75,21 +75,32 @@ hipError_t launch_pack_state(int form, const double* d_illum, int faces, int64_t
                              void* d_state, hipStream_t st);
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor,
                               const SplitPlan& pl, void* d_panel, float* d_row_inv, hipStream_t st);
-// Transport sweep: one direction's level-ordered cells and vec_x0 records on
-// the device (see kernels_sweep.hip).
-struct SweepDirDev {
-    const int32_t* cells;      // [n] cells sorted by level (stable within a level)
-    const int32_t* level_ptr;  // [levels + 1] offsets into cells
-    const int64_t* x0_off;     // [n] first record of each cell (indexed by cell)
-    const uint8_t* state;      // [n] face-state bits (bit f set = in face)
-    const double* s;           // [records] distance back to the in face
-    const uint8_t* in_face;    // [records] in-face id
-    int32_t levels;
-    int32_t pad;
+// Transport sweep (kernels_sweep.hip). Per direction, cells are grouped in
+// levels (a cell reads only from earlier levels) and each level is cut into
+// passes of kSweepBlock cells, one cell per thread of the direction's
+// workgroup. Everything a pass needs that does not depend on earlier results is
+// stored per position in level order, so that it can be fetched a pass ahead.
+constexpr int kSweepBlock = 512;
+constexpr int kSweepRecStride = 10;  // record slots per position (9 used; 80 bytes = five 16-byte loads)
+struct SweepPos {
+    int32_t cell;
+    // bits 0-3: in faces; 4-7: faces out on both sides whose neighbour came
+    // earlier (this cell also writes the neighbour's entry); 8 + 2q: in-face id
+    // of record slot q (q = 3 r + j: node j of the r-th out face)
+    uint32_t meta;
+    int32_t pr[4];  // pairs of the cell
+    int32_t pad[2];
 };
-hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, const int32_t* d_pairs,
-                        const double* d_nodes, const double* d_S, int64_t ld, const double* d_old, double* d_new,
-                        double* d_norm, hipStream_t st);
+struct SweepDirDev {
+    const SweepPos* pos;   // [n] by position in level order
+    const double* s9;      // [n][kSweepRecStride] s of record slot q (0 where unused)
+    const int32_t* pass;   // [passes][2]: first position, count | (last pass of its level) << 31
+    int32_t passes;
+    int32_t levels;
+};
+hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, const double* d_nodes,
+                        const double* d_S, int64_t ld, const double* d_old, double* d_new, double* d_norm,
+                        hipStream_t st);
 // Resident-panel kernel (cell-level state only): plan and launch. Its result
 // stores address 16 output rows through one 32-bit buffer range, so the output
 // leading dimension must stay below kSplitResidentMaxLd cells.

@@ -21,9 +21,8 @@ struct rsh_sweep_s {
     int device = 0;
     int64_t n = 0;
     int m = 0;
-    int32_t* d_pairs = nullptr;
     double* d_nodes = nullptr;
-    std::vector<int32_t> pairs;  // host copy for validation and levels
+    std::vector<int32_t> pairs;  // host copy: validation, levels, per-position copies
     struct Dir {
         void* block = nullptr;  // one device allocation per direction
         SweepDirDev dev{};
@@ -67,11 +66,8 @@ int rsh_sweep_create(int32_t device, int64_t n_cells, int32_t n_dirs, const int3
         return rc;
     };
     if (hipSetDevice(device) != hipSuccess) return cleanup(fail(RSH_E_HIP, "hipSetDevice failed"));
-    if (hipMalloc(&s->d_pairs, (size_t)n_cells * 4 * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&s->d_table, (size_t)n_dirs * sizeof(SweepDirDev)) != hipSuccess)
+    if (hipMalloc(&s->d_table, (size_t)n_dirs * sizeof(SweepDirDev)) != hipSuccess)
         return cleanup(fail(RSH_E_NOMEM, "sweep: device allocation failed"));
-    if (hipMemcpy(s->d_pairs, pairs, (size_t)n_cells * 4 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(RSH_E_HIP, "sweep: pairs upload failed"));
     if (nodes) {
         const size_t b = (size_t)n_cells * 36 * sizeof(double);
         if (hipMalloc(&s->d_nodes, b) != hipSuccess) return cleanup(fail(RSH_E_NOMEM, "sweep: node allocation failed"));
@@ -84,12 +80,11 @@ int rsh_sweep_create(int32_t device, int64_t n_cells, int32_t n_dirs, const int3
 
 int rsh_sweep_destroy(rsh_sweep s) {
     if (!s) return RSH_OK;
-    hipSetDevice(s->device);
+    (void)hipSetDevice(s->device);
     for (auto& d : s->dirs)
-        if (d.block) hipFree(d.block);
-    if (s->d_pairs) hipFree(s->d_pairs);
-    if (s->d_nodes) hipFree(s->d_nodes);
-    if (s->d_table) hipFree(s->d_table);
+        if (d.block) (void)hipFree(d.block);
+    if (s->d_nodes) (void)hipFree(s->d_nodes);
+    if (s->d_table) (void)hipFree(s->d_table);
     delete s;
     return RSH_OK;
 }
@@ -140,51 +135,72 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
         levels = std::max(levels, lv + 1);
     }
     if (r != n_records) return fail(RSH_E_ARG, "more x0 records than out-face nodes");
-    // Cells by level, stable in the given order (counting sort).
+    // Cells by level, ascending cell number inside a level (cells of one level
+    // are independent; neighbouring numbers share cache lines of the per-cell
+    // arrays the pass reads and writes).
     std::vector<int32_t> level_ptr((size_t)levels + 1, 0), cells((size_t)n);
     for (int64_t c = 0; c < n; c++) level_ptr[(size_t)level[(size_t)c] + 1]++;
     for (int32_t l = 0; l < levels; l++) level_ptr[(size_t)l + 1] += level_ptr[(size_t)l];
     std::vector<int32_t> fill(level_ptr.begin(), level_ptr.end() - 1);
-    for (int64_t h = 0; h < n; h++) {
-        const int32_t c = order[h];
-        cells[(size_t)fill[(size_t)level[(size_t)c]]++] = c;
+    for (int64_t c = 0; c < n; c++) cells[(size_t)fill[(size_t)level[(size_t)c]]++] = (int32_t)c;
+    // Per position in the level order: cell, face states, write-back faces (out
+    // on both sides with the neighbour earlier in the given order: the later
+    // writer wins both entries, as in the sequential walk), the in-face ids of
+    // the cell's records, its pairs; and the records' s values.
+    std::vector<SweepPos> posv((size_t)n);
+    std::vector<double> s9((size_t)n * kSweepRecStride, 0.0);
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t c = cells[(size_t)i];
+        const int st = face_state[c];
+        uint32_t meta = (uint32_t)st;
+        SweepPos& P = posv[(size_t)i];
+        P.cell = c;
+        P.pad[0] = P.pad[1] = 0;
+        int q = 0;
+        for (int f = 0; f < 4; f++) {
+            const int32_t p = s->pairs[(size_t)(c * 4 + f)];
+            P.pr[f] = p;
+            if ((st >> f) & 1) continue;
+            if (p >= 0) {
+                const int32_t u = p / 4;
+                if (!((face_state[u] >> (p % 4)) & 1) && pos[(size_t)u] < pos[(size_t)c]) meta |= 1u << (4 + f);
+            }
+            for (int j = 0; j < 3; j++, q++) {
+                const int64_t r = x0_off[(size_t)c] + q;  // records of the cell in out-face order
+                s9[(size_t)(i * kSweepRecStride + q)] = x0_s[r];
+                meta |= (uint32_t)x0_face[r] << (8 + 2 * q);
+            }
+        }
+        P.meta = meta;
     }
-    std::vector<uint8_t> st8((size_t)n);
-    for (int64_t c = 0; c < n; c++) st8[(size_t)c] = (uint8_t)face_state[c];
+    std::vector<int32_t> pass;
+    for (int32_t l = 0; l < levels; l++)
+        for (int32_t b0 = level_ptr[(size_t)l]; b0 < level_ptr[(size_t)l + 1]; b0 += kSweepBlock) {
+            const int32_t cnt = std::min(kSweepBlock, level_ptr[(size_t)l + 1] - b0);
+            const bool last = b0 + kSweepBlock >= level_ptr[(size_t)l + 1];
+            pass.push_back(b0);
+            pass.push_back(cnt | (last ? (int32_t)0x80000000u : 0));
+        }
+    const int32_t passes = (int32_t)(pass.size() / 2);
 
-    const size_t b_cells = align256((size_t)n * 4), b_lp = align256(((size_t)levels + 1) * 4),
-                 b_off = align256((size_t)n * 8), b_st = align256((size_t)n), b_s = align256((size_t)n_records * 8),
-                 b_f = align256((size_t)std::max<int64_t>(n_records, 1));
+    const size_t b_pos = align256((size_t)n * sizeof(SweepPos)), b_s9 = align256((size_t)n * kSweepRecStride * 8),
+                 b_pass = align256(pass.size() * 4);
     std::lock_guard<std::mutex> lock(s->mu);
     RSH_HIP_TRY(hipSetDevice(s->device));
     auto& d = s->dirs[(size_t)k];
     if (d.block) RSH_HIP_TRY(hipFree(d.block));
     d.block = nullptr;
     d.set = false;
-    if (hipMalloc(&d.block, b_cells + b_lp + b_off + b_st + b_s + b_f) != hipSuccess)
+    if (hipMalloc(&d.block, b_pos + b_s9 + b_pass) != hipSuccess)
         return fail(RSH_E_NOMEM, "sweep: direction allocation failed");
     char* base = static_cast<char*>(d.block);
-    auto put = [&](size_t off, const void* src, size_t bytes) {
-        return bytes ? hipMemcpy(base + off, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
-    };
-    size_t off = 0;
-    RSH_HIP_TRY(put(off, cells.data(), (size_t)n * 4));
-    d.dev.cells = reinterpret_cast<const int32_t*>(base + off);
-    off += b_cells;
-    RSH_HIP_TRY(put(off, level_ptr.data(), ((size_t)levels + 1) * 4));
-    d.dev.level_ptr = reinterpret_cast<const int32_t*>(base + off);
-    off += b_lp;
-    RSH_HIP_TRY(put(off, x0_off.data(), (size_t)n * 8));
-    d.dev.x0_off = reinterpret_cast<const int64_t*>(base + off);
-    off += b_off;
-    RSH_HIP_TRY(put(off, st8.data(), (size_t)n));
-    d.dev.state = reinterpret_cast<const uint8_t*>(base + off);
-    off += b_st;
-    RSH_HIP_TRY(put(off, x0_s, (size_t)n_records * 8));
-    d.dev.s = reinterpret_cast<const double*>(base + off);
-    off += b_s;
-    RSH_HIP_TRY(put(off, x0_face, (size_t)n_records));
-    d.dev.in_face = reinterpret_cast<const uint8_t*>(base + off);
+    RSH_HIP_TRY(hipMemcpy(base, posv.data(), (size_t)n * sizeof(SweepPos), hipMemcpyHostToDevice));
+    RSH_HIP_TRY(hipMemcpy(base + b_pos, s9.data(), (size_t)n * kSweepRecStride * 8, hipMemcpyHostToDevice));
+    RSH_HIP_TRY(hipMemcpy(base + b_pos + b_s9, pass.data(), pass.size() * 4, hipMemcpyHostToDevice));
+    d.dev.pos = reinterpret_cast<const SweepPos*>(base);
+    d.dev.s9 = reinterpret_cast<const double*>(base + b_pos);
+    d.dev.pass = reinterpret_cast<const int32_t*>(base + b_pos + b_s9);
+    d.dev.passes = passes;
     d.dev.levels = levels;
     d.records = n_records;
     d.set = true;
@@ -218,8 +234,8 @@ int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t 
         RSH_HIP_TRY(hipMemcpy(s->d_table, table.data(), table.size() * sizeof(SweepDirDev), hipMemcpyHostToDevice));
         s->table_dirty = false;
     }
-    RSH_HIP_TRY(launch_sweep(cls, s->d_table, s->m, s->n, s->d_pairs, s->d_nodes, d_scattering, ld, d_illum_old,
-                             d_illum_new, d_norm, st));
+    RSH_HIP_TRY(launch_sweep(cls, s->d_table, s->m, s->n, s->d_nodes, d_scattering, ld, d_illum_old, d_illum_new,
+                             d_norm, st));
     return RSH_OK;
 }
 
