@@ -268,6 +268,14 @@ typedef struct rsh_batch_s* rsh_batch;
 int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes);
 int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_batch* out);
 int rsh_batch_destroy(rsh_batch b);
+/* Kernel of the split-form slots added from now on: 0 = auto (the resident-
+ * panel grouped kernel where its plan fits one XCD and is specialised: M = 200,
+ * 400, 800 with M <= 800 and n_cells < 2^24; else the streaming-panel one),
+ * 1 = resident panel only (rsh_batch_add fails where it does not apply),
+ * 2 = streaming panel. Both give bit-identical states and residuals. No
+ * reference counterpart (batching is an extension). */
+int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
+
 /* Reserves one device arena of `bytes` for the batch's slot buffers (the batch
  * must be empty): later adds carve their buffers from it first-fit instead of
  * calling hipMalloc / hipFree (which synchronise the device). An add that finds

@@ -98,6 +98,7 @@ PROTOTYPES = {
     "rsh_batch_add": (C.c_int, [vp, vp, i64, vp, vp, f64, i32, P_i32]),
     "rsh_batch_add_seeded": (C.c_int, [vp, vp, i64, u64, f64, i32, P_i32]),
     "rsh_batch_reserve": (C.c_int, [vp, i64]),
+    "rsh_batch_set_split_kernel": (C.c_int, [vp, i32]),
     "rsh_batch_remove": (C.c_int, [vp, i32]),
     "rsh_batch_run": (C.c_int, [vp, i32]),
     "rsh_batch_sync": (C.c_int, [vp]),

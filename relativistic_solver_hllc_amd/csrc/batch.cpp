@@ -28,6 +28,7 @@ using namespace rsh;
 namespace {
 
 constexpr int kMaxClasses = 16;
+constexpr int kXcdLists = 8;  // per-XCD work lists of the resident-panel split classes
 
 struct Slot {
     bool active = false;
@@ -43,6 +44,7 @@ struct Slot {
     size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I, state, cell_inv
     F64Plan plan{};
     SplitPlan splan{};
+    bool res = false;  // split form on the resident-panel grouped kernel
     int iterations = 0;
     int64_t bytes = 0;
 };
@@ -57,6 +59,10 @@ struct ClassState {
     int* d_n_items = nullptr;
     int n_items = 0;
     int counter_index = 0;
+    // resident-panel split classes: K-steps, per-XCD list begins and counts
+    bool res = false;
+    int steps = 0;
+    int* d_meta = nullptr;
 };
 
 }  // namespace
@@ -69,7 +75,8 @@ struct rsh_batch_s {
     std::vector<BatchProblem> h_probs;
     BatchProblem* d_probs = nullptr;
     unsigned long long* d_resid = nullptr;  // 2 * capacity
-    unsigned* d_counters = nullptr;         // kMaxClasses
+    unsigned* d_counters = nullptr;         // kMaxClasses * kXcdLists
+    int split_kernel = 0;                   // 0 auto, 1 resident panel, 2 streaming panel
     std::map<std::pair<int, int>, ClassState> classes;  // by (form, row-block class)
     bool lists_dirty = true;
     bool graph_valid = false;
@@ -147,6 +154,10 @@ void free_slot(rsh_batch b, Slot& s) {
 }
 
 int slot_rb(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.rb_per_pass : s.splan.rb_per_pass; }
+// Class key: (form, row blocks per pass) for the FP64 and streaming split
+// kernels; resident-panel split classes also separate by K-steps (one kernel
+// instance per plan).
+int slot_class(const Slot& s) { return s.res ? 1000 + 64 * s.splan.rb_per_pass + s.splan.steps : slot_rb(s); }
 int slot_passes(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.passes : s.splan.passes; }
 
 void destroy_graph(rsh_batch b) {
@@ -168,10 +179,12 @@ int rebuild(rsh_batch b) {
         if (b->slots[i].active) order.push_back(i);
     std::sort(order.begin(), order.end(), [&](int x, int y) { return b->slots[x].m > b->slots[y].m; });
     bool new_class = false;
+    std::map<std::pair<int, int>, std::vector<std::vector<BatchItem>>> xlists;  // resident classes: per XCD
+    std::map<std::pair<int, int>, int> xnext;                                   // round-robin XCD of the next chunk
     for (int i : order) {
         const Slot& s = b->slots[i];
         const int rb = slot_rb(s);
-        const auto key = std::make_pair(s.form, rb);
+        const auto key = std::make_pair(s.form, slot_class(s));
         if (!b->classes.count(key)) {
             if ((int)b->classes.size() >= kMaxClasses) return fail(RSH_E_UNSUPPORTED, "too many row-block classes");
             ClassState cs;
@@ -180,14 +193,34 @@ int rebuild(rsh_batch b) {
             if (s.form == RSH_FORM_FP64) {
                 cs.tile = f64_tile_for(b->device, rb);
                 cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
+            } else if (s.res) {
+                cs.res = true;
+                cs.steps = s.splan.steps;
+                cs.tile = rsh_tile{8, 1};
+                cs.grid = device_cus(b->device) / kXcdLists * kXcdLists;  // one workgroup per CU (LDS-bound)
+                RSH_HIP_TRY(launch_grouped_iter_split_res(false, cs.form, rb, cs.steps, 0, nullptr, nullptr, nullptr,
+                                                          nullptr, 0, nullptr));
+                RSH_HIP_TRY(hipMalloc(&cs.d_meta, 2 * kXcdLists * sizeof(int)));
             } else {
                 cs.tile = rsh_tile{grouped_iter_split_waves(), 1};
                 cs.grid = grouped_iter_split_resident(s.form, rb, b->device);
             }
-            cs.counter_index = (int)b->classes.size();
+            cs.counter_index = (int)b->classes.size() * kXcdLists;
             RSH_HIP_TRY(hipMalloc(&cs.d_n_items, sizeof(int)));
             b->classes[key] = cs;
             new_class = true;
+        }
+        if (s.res) {
+            // items (problem, row group, chunk); a chunk's row groups go to one XCD's list
+            auto& X = xlists[key];
+            X.resize(kXcdLists);
+            const int64_t cc = iter_split_res_chunk_cells();
+            const int64_t chunks = (s.n + cc - 1) / cc;
+            for (int64_t c = 0; c < chunks; c++) {
+                auto& L = X[(size_t)(xnext[key]++ % kXcdLists)];
+                for (int rg = 0; rg < s.splan.passes; rg++) L.push_back(BatchItem{i, rg, c});
+            }
+            continue;
         }
         const int tiles_w = 16 * b->classes[key].tile.waves;
         const int64_t tiles = (s.n + tiles_w - 1) / tiles_w;
@@ -198,6 +231,16 @@ int rebuild(rsh_batch b) {
     for (auto& kv : b->classes) {
         ClassState& cs = kv.second;
         auto& L = lists[kv.first];
+        int meta[2 * kXcdLists] = {};
+        if (cs.res) {
+            auto& X = xlists[kv.first];
+            X.resize(kXcdLists);
+            for (int x = 0; x < kXcdLists; x++) {
+                meta[x] = (int)L.size();
+                meta[kXcdLists + x] = (int)X[(size_t)x].size();
+                L.insert(L.end(), X[(size_t)x].begin(), X[(size_t)x].end());
+            }
+        }
         if ((int64_t)L.size() > cs.capacity) {
             hipFree(cs.d_items);
             cs.d_items = nullptr;
@@ -209,6 +252,7 @@ int rebuild(rsh_batch b) {
             RSH_HIP_TRY(hipMemcpy(cs.d_items, L.data(), L.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
         cs.n_items = (int)L.size();
         RSH_HIP_TRY(hipMemcpy(cs.d_n_items, &cs.n_items, sizeof(int), hipMemcpyHostToDevice));
+        if (cs.res) RSH_HIP_TRY(hipMemcpy(cs.d_meta, meta, sizeof(meta), hipMemcpyHostToDevice));
     }
     RSH_HIP_TRY(hipMemcpy(b->d_probs, b->h_probs.data(), b->capacity * sizeof(BatchProblem), hipMemcpyHostToDevice));
     if (new_class) destroy_graph(b);
@@ -217,10 +261,15 @@ int rebuild(rsh_batch b) {
 }
 
 int enqueue_iteration(rsh_batch b, int parity) {
-    RSH_HIP_TRY(launch_batch_zero(b->d_probs, b->capacity, b->d_counters, kMaxClasses, parity, b->stream));
+    RSH_HIP_TRY(launch_batch_zero(b->d_probs, b->capacity, b->d_counters, kMaxClasses * kXcdLists, parity,
+                                  b->stream));
     for (auto& kv : b->classes) {
         ClassState& cs = kv.second;
-        if (cs.form == RSH_FORM_FP64)
+        if (cs.res)
+            RSH_HIP_TRY(launch_grouped_iter_split_res(true, cs.form, cs.rb, cs.steps, cs.grid, b->d_probs, cs.d_items,
+                                                      cs.d_meta, b->d_counters + cs.counter_index, parity,
+                                                      b->stream));
+        else if (cs.form == RSH_FORM_FP64)
             RSH_HIP_TRY(launch_grouped_iter_f64(cs.rb, cs.tile.waves, cs.tile.chunk_steps, cs.grid, b->d_probs,
                                                 cs.d_items, cs.d_n_items, b->d_counters + cs.counter_index, parity,
                                                 b->stream));
@@ -283,7 +332,7 @@ int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_ba
     hipError_t e;
     if ((e = hipMalloc(&b->d_probs, capacity * sizeof(BatchProblem))) != hipSuccess ||
         (e = hipMalloc(&b->d_resid, 2 * (size_t)capacity * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&b->d_counters, kMaxClasses * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc(&b->d_counters, kMaxClasses * kXcdLists * sizeof(unsigned))) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
         hipFree(b->d_probs);
         hipFree(b->d_resid);
@@ -307,6 +356,7 @@ int rsh_batch_destroy(rsh_batch b) {
     for (auto& kv : b->classes) {
         hipFree(kv.second.d_items);
         hipFree(kv.second.d_n_items);
+        if (kv.second.d_meta) hipFree(kv.second.d_meta);
     }
     hipFree(b->d_probs);
     hipFree(b->d_resid);
@@ -356,6 +406,18 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     s.albedo = albedo;
     s.plan = plan_f64(ds->m, ds->m);
     s.splan = plan_split(ds->m, ds->m);
+    if (form != RSH_FORM_FP64 && b->split_kernel != 2 && n_cells < (int64_t(1) << 24)) {
+        // resident-panel grouped kernel where its plan fits and is specialised
+        SplitPlan rplan{};
+        int sets = 0;
+        if (plan_split_resident(form, ds->m, ds->m, device_cus(b->device), &rplan, &sets) &&
+            iter_split_res_supported(form, rplan.rb_per_pass, rplan.steps)) {
+            s.splan = rplan;
+            s.res = true;
+        } else if (b->split_kernel == 1) {
+            return fail(RSH_E_UNSUPPORTED, "resident-panel iteration kernel: no specialised plan for this M");
+        }
+    }
     const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
     const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
     s.fbytes = fbytes;
@@ -410,7 +472,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         return hip_fail(e, "batch slot initialisation");
     }
     const PanelEntry* pe = nullptr;
-    int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe);
+    int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe, s.res ? &s.splan : nullptr);
     if (rc) {
         free_slot(b, s);
         return rc;
@@ -459,6 +521,12 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
 int rsh_batch_add_seeded(rsh_batch b, rsh_dirset ds, int64_t n_cells, uint64_t seed, double albedo, int32_t form,
                          int32_t* slot) {
     return add_impl(b, ds, n_cells, albedo, form, nullptr, nullptr, &seed, slot);
+}
+
+int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind) {
+    if (!b || kind < 0 || kind > 2) return fail(RSH_E_ARG, "split kernel: 0 (auto), 1 (resident panel), 2 (streaming)");
+    b->split_kernel = kind;
+    return RSH_OK;
 }
 
 int rsh_batch_reserve(rsh_batch b, int64_t bytes) {

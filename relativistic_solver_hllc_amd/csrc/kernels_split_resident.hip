@@ -36,6 +36,7 @@ constexpr int kResTiles = 2;   // 16-cell tiles per wave
 constexpr int kResChunkTiles = kResWaves * kResTiles;  // 256 cells per chunk
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kResLdsBytes = 160 * 1024;
+constexpr int kIterResSub = 4;  // 256-cell chunks per item of the grouped iteration kernel
 // Buffer descriptor word 3 for raw (untyped) gfx9 buffer access: 32-bit data
 // format, so that the hardware does not treat the descriptor as invalid.
 constexpr int kBufWord3 = 0x00020000;
@@ -55,6 +56,46 @@ __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (I < N) {
         f(std::integral_constant<int, I>{});
         static_for<I + 1, N>(f);
+    }
+}
+
+// One K-step of MFMAs for kResTiles tiles x RBW row blocks: A fragments from
+// the LDS panel (a_base: this lane's vector of the step's first fragment), B
+// from the state vectors in bx. A fragments of one part at a time (part-major),
+// so that only RBW fragments are live. Product-major order: each round touches
+// every (row block, tile) accumulator once, so an MFMA never waits for the
+// previous result of its own accumulator. Products and their order per
+// accumulator are those of the streaming kernel (main hi*hi; cross hi*lo +
+// lo*hi, for three parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi).
+template <int RBW, int FMT, int NT>
+__device__ __forceinline__ void res_mfma_step(const v4u* a_base, const typename Fmt<FMT>::V (&bx)[kResTiles][NT],
+                                              v4f (&accm)[RBW][kResTiles], v4f (&accx)[RBW][kResTiles]) {
+    typedef typename Fmt<FMT>::V V;
+    V a[RBW];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NT + p) * 64]);
+    };
+    auto round = [&](v4f (&acc)[RBW][kResTiles], int pb) {
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
+    };
+    fetch(0);
+    round(accm, 0);
+    if constexpr (NT == 2) {
+        round(accx, 1);
+        fetch(1);
+        round(accx, 0);
+    } else {
+        round(accx, 2);
+        round(accx, 1);
+        fetch(1);
+        round(accx, 1);
+        round(accx, 0);
+        fetch(2);
+        round(accx, 0);
     }
 }
 
@@ -169,37 +210,7 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
 #pragma unroll
                 for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
             if (t + XD < steps) load_x(slot_tag, t + XD);
-            const v4u* a_base = lds + (int64_t)t * RBW * NT * 64 + lane;
-            // A fragments of one part at a time (part-major), so that only RBW
-            // fragments are live. Product-major order: each round touches every
-            // (row block, tile) accumulator once, so an MFMA never waits for the
-            // previous result of its own accumulator.
-            V a[RBW];
-            auto fetch = [&](int p) {
-#pragma unroll
-                for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NT + p) * 64]);
-            };
-            auto round = [&](v4f (&acc)[RBW][kResTiles], int pb) {
-#pragma unroll
-                for (int i = 0; i < RBW; i++)
-#pragma unroll
-                    for (int q = 0; q < kResTiles; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
-            };
-            fetch(0);
-            round(accm, 0);
-            if constexpr (NT == 2) {
-                round(accx, 1);
-                fetch(1);
-                round(accx, 0);
-            } else {
-                round(accx, 2);
-                round(accx, 1);
-                fetch(1);
-                round(accx, 1);
-                round(accx, 0);
-                fetch(2);
-                round(accx, 0);
-            }
+            res_mfma_step<RBW, FMT, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, accm, accx);
         };
         if constexpr (STEPS > 0) {
             static_for<0, STEPS>([&](auto t) { kstep(std::integral_constant<int, t.value % XD>{}, t.value); });
@@ -291,6 +302,208 @@ hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* s
     return hipGetLastError();
 }
 
+// Source iteration I <- (1 - a) B + a S(I) on split state for a batch of
+// problems, resident-panel form (the grouped counterpart of the kernel above).
+// Work: items (problem, row group, kIterResSub chunks of 256 cells) in eight
+// per-XCD lists; workgroup b takes items from list b % 8, so the row groups of
+// one chunk run on one XCD at about the same time and share its L2 copy of the
+// state. Per item the row group's panel slice (all K-steps) and row scales go
+// into LDS, then the chunks are swept as in split_resident_kernel (state
+// prefetch across chunk boundaries, issued before the stores). The epilogue is
+// that of the streaming grouped kernel (kernels_iter_split.hip): S in FP64 from
+// the two accumulators and the exact scales, I_new = (1 - a) B + a S, the
+// residual against the state the MFMAs consumed, I_new re-split into the other
+// ping-pong buffer. Same products, same order: bit-identical to it.
+template <int RBW, int FMT, int NT, int STEPS>
+__global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
+    const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, const int* __restrict__ meta,
+    unsigned* __restrict__ counters, int parity) {
+    typedef typename Fmt<FMT>::V V;
+    typedef typename Fmt<FMT>::T T;
+    constexpr int XD = NT == 2 ? 3 : 2;  // shallower than the plain kernel: the epilogue needs the registers
+    constexpr int steps = STEPS;
+    constexpr int64_t units = (int64_t)steps * RBW * NT * 64;
+    extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then 16 RBW row scales
+    __shared__ int s_item;
+    float* rscale = reinterpret_cast<float*>(lds + units);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const int xcd = blockIdx.x % kXcds;
+    const int begin = meta[xcd], count = meta[kXcds + xcd];
+    constexpr int64_t tile_units = (int64_t)steps * NT * 64;
+    constexpr int tile_bytes = (int)(tile_units * 16);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counters + xcd, 1u);
+        __syncthreads();  // also: every wave is done with the previous item's LDS panel
+        const int it = s_item;
+        __syncthreads();
+        if (it >= count) break;
+        const BatchItem item = items[begin + it];
+        const BatchProblem& P = probs[item.problem];
+        const int rg = item.pass;
+        const int row0 = rg * RBW * 16;
+        const v4u* __restrict__ src = static_cast<const v4u*>(P.spanel) + (int64_t)rg * units;
+        for (int64_t u = threadIdx.x; u < units; u += 64 * kResWaves) lds[u] = src[u];
+        for (int r = threadIdx.x; r < RBW * 16; r += 64 * kResWaves)
+            rscale[r] = row0 + r < P.m ? P.row_inv[row0 + r] : 0.0f;
+        __syncthreads();
+
+        const v4u* __restrict__ st_in = static_cast<const v4u*>(P.state[parity]);
+        v4u* __restrict__ st_out = static_cast<v4u*>(P.state[parity ^ 1]);
+        const int64_t n = P.n;
+        const int64_t tiles = (n + 15) / 16;
+        const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
+        const int64_t c0 = item.tile * kIterResSub;
+        const int64_t c1 = c0 + kIterResSub < chunks ? c0 + kIterResSub : chunks;
+
+        __amdgpu_buffer_rsrc_t xs;
+        int xoff[kResTiles];
+        int64_t ntile_of_bound = 0;
+        auto bind_chunk = [&](int64_t chunk) {
+            const int64_t ct_chunk = chunk * kResChunkTiles;
+            const int ntile = (int)(tiles - ct_chunk < kResChunkTiles ? tiles - ct_chunk : kResChunkTiles);
+            ntile_of_bound = ntile;
+            xs = __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(st_in + ct_chunk * tile_units), (short)0,
+                                                   ntile * tile_bytes, kBufWord3);
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) {
+                const int tile = wave * kResTiles + q;
+                xoff[q] = (tile < ntile ? tile : ntile - 1) * tile_bytes + lane * 16;
+            }
+        };
+        v4u xr[XD][kResTiles][NT];
+        auto load_x = [&](auto slot_tag, int t) {
+            constexpr int sl = decltype(slot_tag)::value;
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++)
+#pragma unroll
+                for (int p = 0; p < NT; p++)
+                    xr[sl][q][p] = __builtin_bit_cast(
+                        v4u, __builtin_amdgcn_raw_buffer_load_b128(xs, xoff[q] + p * 1024, t * NT * 1024, 0));
+        };
+        auto prefetch_head = [&]() {
+            static_for<0, XD>([&](auto u) {
+                if (u.value < steps) load_x(u, u.value);
+            });
+        };
+        double res = 0.0;
+        if (c0 < c1) {
+            bind_chunk(c0);
+            prefetch_head();
+        }
+        for (int64_t chunk = c0; chunk < c1; chunk++) {
+            const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
+            float cinv[kResTiles];
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) {
+                const int64_t cell = (ct0 + q) * 16 + cl;
+                cinv[q] = P.cell_inv[cell < n ? cell : n - 1];
+            }
+            v4f accm[RBW][kResTiles], accx[RBW][kResTiles];
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
+#pragma unroll
+                for (int q = 0; q < kResTiles; q++) {
+                    accm[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
+                    accx[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
+                }
+            static_for<0, STEPS>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
+                constexpr int sl = t % XD;
+                __builtin_amdgcn_sched_barrier(0);
+                V bx[kResTiles][NT];
+#pragma unroll
+                for (int q = 0; q < kResTiles; q++)
+#pragma unroll
+                    for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
+                if constexpr (t + XD < STEPS) load_x(std::integral_constant<int, sl>{}, t + XD);
+                res_mfma_step<RBW, FMT, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, accm, accx);
+            });
+            // This chunk's state (old values) and the output chunk through buffer
+            // descriptors (uniform base, 32-bit offsets), kept before the next
+            // chunk's head is bound and requested ahead of this chunk's stores.
+            const __amdgpu_buffer_rsrc_t xin = xs;
+            const int ntile_cur = (int)ntile_of_bound;
+            const __amdgpu_buffer_rsrc_t xout = __builtin_amdgcn_make_buffer_rsrc(
+                st_out + chunk * kResChunkTiles * tile_units, (short)0, ntile_cur * tile_bytes, kBufWord3);
+            __builtin_amdgcn_sched_barrier(0);
+            if (chunk + 1 < c1) {
+                bind_chunk(chunk + 1);
+                prefetch_head();
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            int zoff;
+            asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
+            const float* __restrict__ rs = rscale + 4 * g + zoff;
+#pragma unroll
+            for (int q = 0; q < kResTiles; q++) {
+                const int tl = wave * kResTiles + q;  // tile within the chunk
+                const int64_t cell = (ct0 + q) * 16 + cl;
+                if (tl >= ntile_cur || cell >= n) continue;
+                const double ci = (double)cinv[q];
+                const double cscale = 1.0 / ci;  // power of two: exact
+                const int cbyte = (int)(cell * 8);
+#pragma unroll
+                for (int i = 0; i < RBW; i++) {
+                    const int rbg = rg * RBW + i;
+                    if (rbg * 16 >= P.m) continue;  // padding row block: no rows, no state
+                    const int s = rbg >> 1;
+                    const int vlane = (2 * (rbg & 1) + (g >> 1)) * 16 + cl;
+                    const int voff = (tl * (int)tile_units + vlane) * 16 + 8 * (g & 1);
+                    const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
+                        const_cast<double*>(P.B) + (int64_t)rbg * 16 * n, (short)0, (int)(16 * n * 8), kBufWord3);
+                    T oldp[NT][4], newp[NT][4];
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        const v2u raw = __builtin_bit_cast(
+                            v2u, __builtin_amdgcn_raw_buffer_load_b64(xin, voff, (s * NT + t) * 1024, 0));
+                        T tmp[4];
+                        __builtin_memcpy(tmp, &raw, 8);
+#pragma unroll
+                        for (int r = 0; r < 4; r++) oldp[t][r] = tmp[r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int k = rbg * 16 + 4 * g + r;
+                        if (k < P.m) {
+                            const double bv = __builtin_bit_cast(
+                                double, __builtin_amdgcn_raw_buffer_load_b64(bsrc, (int)(4 * g * n * 8) + cbyte,
+                                                                             (int)(r * n * 8), 0));
+                            const double S = ((double)accm[i][q][r] + (double)accx[i][q][r]) *
+                                             (double)rs[i * 16 + r] * ci;
+                            const double nv = P.one_minus_albedo * bv + P.albedo * S;
+                            float of = 0.0f;
+#pragma unroll
+                            for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
+                            const double old = (double)of * ci;
+                            res = fmax(res, fabs(old - nv) / nv);
+                            T qp[NT];
+                            split_parts<FMT, NT>((float)(nv * cscale), qp);
+#pragma unroll
+                            for (int t = 0; t < NT; t++) newp[t][r] = qp[t];
+                        } else {
+#pragma unroll
+                            for (int t = 0; t < NT; t++) newp[t][r] = (T)0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        v2u raw;
+                        __builtin_memcpy(&raw, newp[t], 8);
+                        __builtin_amdgcn_raw_buffer_store_b64(raw, xout, voff, (s * NT + t) * 1024, 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) res = fmax(res, __shfl_xor(res, off, 64));
+        if (lane == 0 && res > 0.0)
+            atomicMax(P.resid + parity, (unsigned long long)__double_as_longlong(res));
+    }
+}
+
 // Largest row-block count per workgroup that the compiler keeps in 256
 // registers without spills (two parts: 5; three parts: 4).
 constexpr int res_max_rbw(int nt) { return nt == 2 ? 5 : 4; }
@@ -364,5 +577,44 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
 #undef RSH_RES
 #undef RSH_RES_S
 }
+
+// Grouped iteration, resident-panel form: the specialised plans (M = 200, 400,
+// 800). rb = RBW of the class, steps its K-steps; false if not specialised.
+bool iter_split_res_supported(int form, int rb, int steps) {
+    const int nt = split_terms(form);
+    if (nt == 2) return (rb == 5 && (steps == 7 || steps == 13)) || (rb == 3 && steps == 25);
+    return (rb == 4 && (steps == 7 || steps == 13)) || (rb == 2 && steps == 25);
+}
+
+size_t iter_split_res_lds(int form, int rb, int steps) {
+    return (size_t)steps * rb * split_terms(form) * 1024 + (size_t)rb * 16 * 4;
+}
+
+template <int RBW, int FMT, int NT, int STEPS>
+static hipError_t iter_res_one(bool launch, int grid, size_t lds, hipStream_t st, const BatchProblem* probs,
+                               const BatchItem* items, const int* meta, unsigned* counters, int parity) {
+    auto kern = grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS>;
+    if (!launch)
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kResWaves), lds, st, probs, items, meta, counters, parity);
+    return hipGetLastError();
+}
+
+// launch = false: only sets the kernel's dynamic-LDS limit (call once, outside graph capture).
+hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int grid,
+                                         const BatchProblem* d_probs, const BatchItem* d_items, const int* d_meta,
+                                         unsigned* d_counters, int parity, hipStream_t st) {
+    const int fmt = split_fmt(form), nt = split_terms(form);
+    const size_t lds = iter_split_res_lds(form, rb, steps);
+#define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
+    RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(3, 0, 2, 25)
+    RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(3, 1, 2, 25)
+    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(2, 0, 3, 25)
+#undef RSH_IR
+    return hipErrorInvalidValue;
+}
+
+int iter_split_res_chunk_cells() { return kIterResSub * kResChunkTiles * 16; }
 
 }  // namespace rsh

@@ -150,6 +150,15 @@ hipError_t launch_grouped_iter_split(int form, int rb, int grid, const BatchProb
                                      const BatchItem* d_items, const int* d_n_items, unsigned* d_counter, int parity,
                                      hipStream_t st);
 int grouped_iter_split_resident(int form, int rb, int device);
+// Grouped iteration on split state, resident-panel form (kernels_split_resident.hip):
+// items (problem, row group, chunk of iter_split_res_chunk_cells() cells) in 8
+// per-XCD lists; meta = [8 list begins, 8 list counts]; counters = 8 per class.
+bool iter_split_res_supported(int form, int rb, int steps);
+size_t iter_split_res_lds(int form, int rb, int steps);
+hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int grid,
+                                         const BatchProblem* d_probs, const BatchItem* d_items, const int* d_meta,
+                                         unsigned* d_counters, int parity, hipStream_t st);
+int iter_split_res_chunk_cells();
 int grouped_iter_split_waves();
 hipError_t launch_unpack_state(int form, const void* d_state, int64_t n, int m, const float* d_cell_inv,
                                double* d_out, hipStream_t st);

@@ -275,6 +275,10 @@ class Batch:
                FORMS[form], C.byref(slot))
         return slot.value
 
+    def set_split_kernel(self, kind: str) -> None:
+        """Kernel for split-form slots added afterwards: "auto", "resident" or "streaming"."""
+        N.call("rsh_batch_set_split_kernel", self.handle, {"auto": 0, "resident": 1, "streaming": 2}[kind])
+
     def reserve(self, nbytes: int) -> None:
         """One device arena for the slot buffers (batch must be empty)."""
         N.call("rsh_batch_reserve", self.handle, int(nbytes))
