@@ -35,7 +35,7 @@ sys.path.insert(0, REPO)
 METRIC = "cell·dir pairs/s + %roofline, FP64 vs split-BF16 @≤1e-6 rel; serve p99 ms"
 FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(REPO, "profiles", "pmc_mfma_f64_config2.json")
+PMC_SUMMARY = os.path.join(REPO, "profiles", "pmc_mfma_f64_1572864x400.json")
 
 
 def log(msg: str) -> None:
@@ -70,11 +70,11 @@ def cpu_baseline(illum_dev, xyz, w, F, n, m, threads, k_slice, variant):
 
 
 def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cells):
-    """Config 3: same mesh and intensities, HG g = 0.8 with FP64 row normalisation;
+    """Henyey-Greenstein workload: same mesh and intensities, HG g = 0.8 with FP64 row normalisation;
     FP64 MFMA vs the split forms (state packed once, untimed; FP32 MFMA accumulation)."""
     from oracle import orc
 
-    res = {"workload": "config 3: same 1,572,864-cell x 4-face mesh and intensities, M=400, "
+    res = {"workload": "same 1,572,864-cell x 4-face mesh and intensities, M=400, "
                        "Henyey-Greenstein g=0.8, FP64 row normalisation",
            "phase": "hg", "g": 0.8, "forms": {}}
     ds = rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True)
@@ -91,7 +91,7 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
     ref64 = out.gather_columns(wide)
     P = orc.phase_matrix(xyz, w, F, 1, 0.8, True)
     ref_or = orc.scatter_matrix(illum.gather_cells(cells), P, threads=16)
-    log(f"[bench] config3 FP64 HG: {t64:.3f} ms")
+    log(f"[bench] HG g=0.8 FP64: {t64:.3f} ms")
     out32 = D.DeviceArray((m, n), np.float32)
 
     def timed(fn):
@@ -343,7 +343,7 @@ def pcie_section(D, rs, illum, xyz, w, F, n, m):
             "total_ms": total * 1e3, "pairs_per_s_pcie_inclusive": n * m / total, "finite_output": ok}
 
 
-def config4_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
+def uneven_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
     """Uneven problems: N log-uniform in [n_lo, n_hi] rounded to a structured cube 6 n^3,
     M in {200, 400, 800}, phase Thomson or HG g in {0.3, 0.6, 0.9}."""
     rng = np.random.default_rng(seed)
@@ -359,12 +359,9 @@ def config4_problems(seed=0x5EED + 4, count=32, n_lo=1e4, n_hi=5e5):
 
 
 def batch_section(D, rs, iters=20):
-    """Config 4: 32 uneven problems, `iters` source iterations I <- 0.5 B + 0.5 S each,
+    """Batch workload: 32 uneven problems, `iters` source iterations I <- 0.5 B + 0.5 S each,
     state resident on the device, grouped graph-replayed launches; FP64 and split forms."""
-    probs = config4_problems()
-    for m in (200, 400, 800):
-        rs.autotune_f64(m, faces=1, n_cells=65536)
-    tiles = {m: rs.tuned_tile_f64(m) for m in (200, 400, 800)}
+    probs = uneven_problems()
     dirsets = {}
     for p in probs:
         key = (p["m"], p["phase"], p["g"])
@@ -373,11 +370,13 @@ def batch_section(D, rs, iters=20):
             dirsets[key] = rs.DirectionSet(xyz, w, F, phase=p["phase"], g=p["g"], normalize=True)
     pairs = sum(p["n"] * p["m"] for p in probs)
     flops = sum(2.0 * p["m"] ** 2 * p["n"] for p in probs)
-    res = {"workload": "config 4: 32 problems, N log-uniform in [1e4, 5e5] as 6 n^3 cells, M in {200,400,800}, "
+    res = {"workload": "32 problems, N log-uniform in [1e4, 5e5] as 6 n^3 cells, M in {200,400,800}, "
                        "Thomson or HG g in {0.3,0.6,0.9} (FP64 row-normalised), I0 ~ log-normal(0,1), "
                        "B ~ U(0.5,2), albedo 0.5, 20 iterations",
            "problems": len(probs), "cells_total": sum(p["n"] for p in probs), "pairs_per_iteration": pairs,
-           "iterations": iters, "autotuned_fp64_tiles": {str(k): list(v) for k, v in tiles.items()}, "forms": {}}
+           "iterations": iters,
+           "fp64_grouped_tile": "4 waves x 2 j-steps per LDS stage, <= 13 row blocks (208 rows) per pass",
+           "forms": {}}
     states = {}
     for form in ("fp64", "bf16x2", "fp16x2", "bf16x3"):
         b = rs.Batch(len(probs))
@@ -410,7 +409,7 @@ def batch_section(D, rs, iters=20):
             entry["max_rel_err_vs_fp64_state"] = max(errs)
             entry["speedup_vs_fp64"] = res["forms"]["fp64"]["seconds"] / dt
         res["forms"][form] = entry
-        log(f"[bench] config4 {form}: {dt * 1e3 / iters:.2f} ms/iteration, {pairs * iters / dt:.3e} pairs/s")
+        log(f"[bench] batch {form}: {dt * 1e3 / iters:.2f} ms/iteration, {pairs * iters / dt:.3e} pairs/s")
         b.close()
     for ds in dirsets.values():
         ds.close()
@@ -418,13 +417,13 @@ def batch_section(D, rs, iters=20):
 
 
 def serve_section(rs, rate=10.0, count=64, budget_gb=200):
-    """Config 5: real-time Poisson arrivals of seeded requests drawn as in config 4 with
+    """Serving workload: real-time Poisson arrivals of seeded requests drawn as in the batch workload with
     N in [1e4, 1e6]; each iterates to its FP64 residual tolerance; continuous batching."""
-    res = {"workload": f"config 5: Poisson arrivals at {rate:g}/s, {count} requests, N log-uniform in [1e4, 1e6] "
+    res = {"workload": f"Poisson arrivals at {rate:g}/s, {count} requests, N log-uniform in [1e4, 1e6] "
                        "as 6 n^3, M in {200,400,800}, Thomson or HG; tolerance 1e-8 (fp64) / 1e-6 (split); "
                        f"HBM budget {budget_gb} GB, 64 slots, 2 iterations per graph launch",
            "forms": {}}
-    probs = config4_problems(seed=0x5EED + 5, count=count, n_lo=1e4, n_hi=1e6)
+    probs = uneven_problems(seed=0x5EED + 5, count=count, n_lo=1e4, n_hi=1e6)
     rng = np.random.default_rng(0x5EED + 55)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
     for form, tol in (("fp64", 1e-8), ("bf16x3", 1e-6)):
@@ -443,7 +442,7 @@ def serve_section(rs, rate=10.0, count=64, budget_gb=200):
             "converged": sum(inf["status"] == "converged" for inf in infos),
             "mean_iterations": float(np.mean([inf["iterations"] for inf in infos])),
         }
-        log(f"[bench] config5 {form}: p50 {res['forms'][form]['latency_p50_ms']:.1f} ms, "
+        log(f"[bench] serve {form}: p50 {res['forms'][form]['latency_p50_ms']:.1f} ms, "
             f"p99 {res['forms'][form]['latency_p99_ms']:.1f} ms, {count / wall:.1f} req/s")
     return res
 
@@ -460,9 +459,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-slice", type=int, default=16)
     ap.add_argument("--no-moments", action="store_true", help="skip the radiation-moments section")
-    ap.add_argument("--no-split", action="store_true", help="skip the split-precision (config 3) section")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems (config 4) section")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) section")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision (HG g=0.8) section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--no-sweep", action="store_true", help="skip the transport-sweep section")
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) path")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,

@@ -239,8 +239,9 @@ int rsh_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
 /* ------------------------------------------------ tile autotuner */
 /* Times every supported FP64 MFMA tile on a synthetic problem (n_cells cells,
  * m directions, all m output rows) and records the fastest for the row-block
- * class of m; later FP64 launches (RSH_F64_AUTO) and batches of that class use
- * it. No reference counterpart. */
+ * class of m; later single-problem FP64 launches (RSH_F64_AUTO) of that class
+ * use it. Batches use the grouped kernel's own tile (4 waves, 2 j-steps per
+ * stage, at most 13 row blocks per pass). No reference counterpart. */
 int rsh_autotune_f64(int32_t device, int32_t m, int32_t faces, int64_t n_cells, rsh_tile* best, float* best_ms);
 int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
 

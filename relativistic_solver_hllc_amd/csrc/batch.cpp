@@ -3,9 +3,10 @@
 //
 // A batch owns, per problem slot, the ping-pong cell-level state and the
 // source term; the phase panel comes from the problem's direction set. Slots
-// are grouped by row-block class (row blocks per pass of the MFMA kernel:
-// 13 for M = 200, 25 for M = 400 and M = 800); every class has a device work
-// list of (problem, pass, cell tile) items that one persistent launch consumes.
+// are grouped by row-block class (row blocks per pass of the MFMA kernel; the
+// FP64 iteration caps passes at 13 row blocks, so M = 200, 400 and 800 share
+// one class with 1, 2 and 4 passes); every class has a device work list of
+// (problem, pass, cell tile) items that one persistent launch consumes.
 // One batch iteration = a zeroing kernel + one grouped launch per class. Two
 // iterations (parity 0 then 1) are captured once into a hipGraph; changing the
 // active set rewrites the work lists and counts in device memory, which leaves
@@ -191,7 +192,7 @@ int rebuild(rsh_batch b) {
             cs.form = s.form;
             cs.rb = rb;
             if (s.form == RSH_FORM_FP64) {
-                cs.tile = f64_tile_for(b->device, rb);
+                cs.tile = rsh_tile{kGroupedF64Waves, kGroupedF64ChunkSteps};
                 cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
             } else if (s.res) {
                 cs.res = true;
@@ -404,7 +405,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     s.m = ds->m;
     s.form = form;
     s.albedo = albedo;
-    s.plan = plan_f64(ds->m, ds->m);
+    s.plan = plan_f64(ds->m, ds->m, kGroupedF64MaxRb);
     s.splan = plan_split(ds->m, ds->m);
     if (form != RSH_FORM_FP64 && b->split_kernel != 2 && n_cells < (int64_t(1) << 24)) {
         // resident-panel grouped kernel where its plan fits and is specialised
@@ -472,7 +473,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         return hip_fail(e, "batch slot initialisation");
     }
     const PanelEntry* pe = nullptr;
-    int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe, s.res ? &s.splan : nullptr);
+    int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe, s.res ? &s.splan : nullptr, kGroupedF64MaxRb);
     if (rc) {
         free_slot(b, s);
         return rc;

@@ -63,9 +63,9 @@ void free_panel(PanelEntry& pe) {
 
 // Panel for rows [k0,k1), face count and form, built on stream st if absent.
 int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out,
-              const SplitPlan* resident) {
+              const SplitPlan* resident, int f64_max_rb) {
     std::lock_guard<std::mutex> lock(ds->mu);
-    auto key = std::make_tuple(k0, k1, faces, form, resident ? 1 : 0);
+    auto key = std::make_tuple(k0, k1, faces, form, form == RSH_FORM_FP64 ? f64_max_rb : (resident ? 1 : 0));
     auto it = ds->panels.find(key);
     if (it != ds->panels.end()) {
         // Make st wait for the build only while it may still be running.
@@ -83,7 +83,7 @@ int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st
     PanelEntry pe;
     hipError_t e = hipSuccess;
     if (form == RSH_FORM_FP64) {
-        pe.plan = plan_f64(ds->m, k1 - k0);
+        pe.plan = plan_f64(ds->m, k1 - k0, f64_max_rb);
         const int64_t nd = panel_f64_doubles(pe.plan);
         e = hipMalloc(&pe.d_panel, std::max<int64_t>(nd, 1) * sizeof(double));
         if (e == hipSuccess)

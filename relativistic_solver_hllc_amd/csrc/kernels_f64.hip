@@ -93,10 +93,11 @@ __global__ __launch_bounds__(256) void panel_f64_kernel(const double* __restrict
 
 static const int kRbSet[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 13, 16, 20, 25};
 
-F64Plan plan_f64(int m, int rows) {
+F64Plan plan_f64(int m, int rows, int max_rb) {
     F64Plan pl{};
     const int nrb = (rows + 15) / 16;
-    pl.passes = nrb > 0 ? (nrb + kMaxRowBlocksPerPass - 1) / kMaxRowBlocksPerPass : 0;
+    if (max_rb < 1 || max_rb > kMaxRowBlocksPerPass) max_rb = kMaxRowBlocksPerPass;
+    pl.passes = nrb > 0 ? (nrb + max_rb - 1) / max_rb : 0;
     const int per = pl.passes > 0 ? (nrb + pl.passes - 1) / pl.passes : 0;
     pl.rb_per_pass = 0;
     for (int v : kRbSet)

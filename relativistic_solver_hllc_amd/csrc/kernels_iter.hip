@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void batch_zero_kernel(BatchProblem* __restric
 }
 
 template <int RB, int W, int SC>
-__global__ __launch_bounds__(64 * W) void grouped_iter_f64_kernel(const BatchProblem* __restrict__ probs,
+__global__ __launch_bounds__(64 * W, 2) void grouped_iter_f64_kernel(const BatchProblem* __restrict__ probs,
                                                                   const BatchItem* __restrict__ items,
                                                                   const int* __restrict__ n_items,
                                                                   unsigned* __restrict__ counter, int parity) {

@@ -30,6 +30,14 @@ int hip_fail(hipError_t e, const char* what);
 // contiguous bytes, so a wave reads it with one coalesced load / LDS read.
 constexpr int kPanelStepAlign = 4;      // j-steps padded to a multiple of this
 constexpr int kMaxRowBlocksPerPass = 25; // 400 output directions per pass
+// The grouped iteration kernel's fused epilogue (B and the previous state read
+// per output) is latency-bound at one workgroup per CU; passes of at most 13 row
+// blocks (208 rows; 166 VGPRs at 4 waves x 2 j-steps per LDS stage) let three
+// workgroups share a CU, so one's epilogue overlaps the others' MFMA work, for
+// 4 % padding at M = 200, 400 and 800 and a re-read of the state per pass.
+constexpr int kGroupedF64MaxRb = 13;
+constexpr int kGroupedF64Waves = 4;
+constexpr int kGroupedF64ChunkSteps = 2;
 
 struct F64Plan {
     int rb_per_pass;   // row blocks per workgroup pass (template value)
@@ -38,7 +46,7 @@ struct F64Plan {
     int steps;         // padded j-steps (multiple of kPanelStepAlign)
 };
 
-F64Plan plan_f64(int m, int rows);
+F64Plan plan_f64(int m, int rows, int max_rb = kMaxRowBlocksPerPass);
 int64_t panel_f64_doubles(const F64Plan& pl);
 
 // ------------------------------------------------------------------ kernels

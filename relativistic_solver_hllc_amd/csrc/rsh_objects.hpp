@@ -36,9 +36,10 @@ struct rsh_dirset_s {
 
 namespace rsh {
 // Phase panel of (k0, k1, faces, form), built on stream st if absent. A split
-// panel with resident != nullptr uses that row-group plan (layout 1).
+// panel with resident != nullptr uses that row-group plan (layout 1); an FP64
+// panel uses passes of at most f64_max_rb row blocks.
 int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out,
-              const SplitPlan* resident = nullptr);
+              const SplitPlan* resident = nullptr, int f64_max_rb = kMaxRowBlocksPerPass);
 // Compute units of a device (cached).
 int device_cus(int device);
 // Tile of the FP64 MFMA kernel chosen for a row-block class (tuned or default).

@@ -76,8 +76,8 @@ def test_phase_matrix_matches_oracle(gpu):
         assert np.max(np.abs(P.sum(axis=1) - 1)) < 1e-14
 
 
-def test_config1_parity(gpu):
-    """Config 1: 16^3 x 6 = 24,576 cells, M = 200 Fibonacci, Thomson, log-normal(0,1)."""
+def test_24576x200_parity(gpu):
+    """16^3 x 6 = 24,576 cells, M = 200 Fibonacci, Thomson, log-normal(0,1)."""
     n, m = 6 * 16 ** 3, 200
     rng = np.random.default_rng(0x5EED + 1)
     illum = rng.lognormal(0.0, 1.0, size=(m, n, 4))
@@ -88,7 +88,7 @@ def test_config1_parity(gpu):
         assert np.array_equal(Sd, ref), f"direct kernel not bit-exact: max rel {rel(Sd, ref):.3e}"
         Sm = run(ds, illum, 4, 0, m)
         err = rel(Sm, ref)
-        print(f"config1 MFMA max rel err {err:.3e}")
+        print(f"24,576 x 200 MFMA max rel err {err:.3e}")
         assert err < 1e-12
 
 
@@ -204,8 +204,8 @@ def test_problem_sync_copy_and_full_range(gpu):
 
 
 @pytest.mark.big
-def test_config2_full_size(gpu):
-    """Config 2: 64^3 x 6 = 1,572,864 cells, M = 400, Thomson, log-normal(0, 1.5) (20.1 GB of faces)."""
+def test_1572864x400_full_size(gpu):
+    """64^3 x 6 = 1,572,864 cells, M = 400, Thomson, log-normal(0, 1.5) (20.1 GB of faces)."""
     n, m = 6 * 64 ** 3, 400
     xyz, w, F = rs.fibonacci_directions(m)
     illum = D.DeviceArray((m, n, 4))
@@ -224,7 +224,7 @@ def test_config2_full_size(gpu):
             ref = orc.scatter_reference(sub, xyz, w, F, threads=16)
             got = out.gather_columns(cells)
             err = rel(got, ref)
-            print(f"config2 sampled max rel err {err:.3e}")
+            print(f"1,572,864 x 400 sampled max rel err {err:.3e}")
             assert err < 1e-12
             # linearity: doubling the input doubles the output exactly
             illum.scale(2.0)

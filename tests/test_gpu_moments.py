@@ -81,7 +81,7 @@ def test_moments_match_oracle_ragged(gpu, n, fpt):
     check(got, ref, illum, w, F, areas, volume)
 
 
-def test_moments_config1_sizes(gpu):
+def test_moments_24576x200_sizes(gpu):
     """24,576 cells (16^3 x 6), M = 200 Fibonacci directions, log-normal(0, 1) faces."""
     n, m = 6 * 16 ** 3, 200
     rng = np.random.default_rng(77)
@@ -180,7 +180,7 @@ def test_problem_all_param_stream_ordering(gpu):
 
 
 @pytest.mark.big
-def test_moments_config2_full_size(gpu):
+def test_moments_1572864x400_full_size(gpu):
     """1,572,864 cells, M = 400, log-normal(0, 1.5) faces (20.1 GB): sampled cells vs
     the oracle, and linearity (doubling the input doubles every moment exactly)."""
     n, m = 6 * 64 ** 3, 400

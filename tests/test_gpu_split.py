@@ -45,8 +45,8 @@ def run_split(ds, form, illum, faces, k0, k1, out_f32=False, ld=None):
 
 
 @pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
-def test_config3_like_hg08(gpu, form):
-    """Config 3 shape at reduced N: HG g = 0.8, FP64 row normalisation, log-normal(0, 1.5) faces."""
+def test_hg08_reduced_n(gpu, form):
+    """The HG workload's shape at reduced N: HG g = 0.8, FP64 row normalisation, log-normal(0, 1.5) faces."""
     n, m = 4096, 400
     rng = np.random.default_rng(0x5EED + 3)
     illum = rng.lognormal(0.0, 1.5, size=(m, n, 4))

@@ -1,8 +1,8 @@
 """The C++ drop-in of cuda/cuda_solve.h (include/rsh_cuda_solve.hpp).
 
 CPU: every function the header declares is defined by the shim (checked on the
-built driver's symbol table) and the signatures match the reference's.
-GPU: the driver replays the reference's illumination-iteration call sequence
+built test program's symbol table) and the signatures match the reference's.
+GPU: the test program replays the reference's illumination-iteration call sequence
 (Illum_async.cpp:663-1072) through the shim; both the async two-stream result
 and CalculateIntScattering match the oracle, and on the rank with start == 0
 CalculateAllParamStream (with geometry read from normals.bin / squares.bin /
