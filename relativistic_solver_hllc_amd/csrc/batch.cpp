@@ -42,7 +42,8 @@ struct Slot {
     void* state[2] = {nullptr, nullptr}; // split forms
     float* cell_inv = nullptr;
     double* B = nullptr;
-    size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I, state, cell_inv
+    size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I (M rows), state, cell_inv
+    size_t abytes = 0;                          // allocated size of B / I (FP64: padded rows)
     F64Plan plan{};
     SplitPlan splan{};
     bool res = false;  // split form on the resident-panel grouped kernel
@@ -142,12 +143,12 @@ void dev_free(rsh_batch b, void* p, size_t bytes) {
 }
 
 void free_slot(rsh_batch b, Slot& s) {
-    dev_free(b, s.I[0], s.fbytes);
-    dev_free(b, s.I[1], s.fbytes);
+    dev_free(b, s.I[0], s.abytes);
+    dev_free(b, s.I[1], s.abytes);
     dev_free(b, s.state[0], s.sbytes);
     dev_free(b, s.state[1], s.sbytes);
     dev_free(b, s.cell_inv, s.cbytes);
-    dev_free(b, s.B, s.fbytes);
+    dev_free(b, s.B, s.abytes);
     s.I[0] = s.I[1] = nullptr;
     s.state[0] = s.state[1] = nullptr;
     s.cell_inv = nullptr;
@@ -299,8 +300,14 @@ int capture(rsh_batch b) {
     return RSH_OK;
 }
 
+// Rows allocated per FP64 state / source array: the iteration plan's padded
+// row count (whole row blocks). The grouped kernel reads and writes the padding
+// rows (zero, so their residual term is 0 / 0 and ignored) instead of
+// branching per row.
+int64_t f64_rows(int m) { return (int64_t)plan_f64(m, m, kGroupedF64MaxRb).rb_total * 16; }
+
 int64_t footprint(int form, int64_t n, int m) {
-    if (form == RSH_FORM_FP64) return 3 * (int64_t)m * n * 8;  // I ping-pong + B
+    if (form == RSH_FORM_FP64) return 3 * f64_rows(m) * n * 8;  // I ping-pong + B, padded rows
     if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)
         return 2 * split_state_units(form, 1, n, m) * 16 + (int64_t)m * n * 8 + n * 4;  // state ping-pong + B + scales
     return -1;
@@ -380,6 +387,9 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         return fail(RSH_E_ARG, "invalid batch_add arguments");
     if (footprint(form, 1, 1) < 0) return fail(RSH_E_ARG, "unknown form");
     if (ds->device != b->device) return fail(RSH_E_ARG, "direction set lives on another device");
+    // the FP64 iteration kernel's 32-bit buffer offsets span one row block
+    if (form == RSH_FORM_FP64 && n_cells >= kGroupedF64MaxCells)
+        return fail(RSH_E_UNSUPPORTED, "FP64 batch problems hold fewer than 2^24 cells");
     int id = -1;
     for (int i = 0; i < b->capacity; i++)
         if (!b->slots[i].active) {
@@ -422,11 +432,16 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
     const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
     s.fbytes = fbytes;
+    s.abytes = form == RSH_FORM_FP64 ? (size_t)f64_rows(ds->m) * n_cells * sizeof(double) : fbytes;
     s.sbytes = sbytes;
     s.cbytes = form == RSH_FORM_FP64 ? 0 : (size_t)n_cells * sizeof(float);
-    hipError_t e = dev_alloc(b, fbytes, (void**)&s.B);
+    hipError_t e = dev_alloc(b, s.abytes, (void**)&s.B);
     if (e == hipSuccess && form == RSH_FORM_FP64) {
-        if ((e = dev_alloc(b, fbytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, fbytes, (void**)&s.I[1]);
+        if ((e = dev_alloc(b, s.abytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, s.abytes, (void**)&s.I[1]);
+        // zero padding rows: the iteration kernel reads them and writes zeros back
+        for (double* a : {s.B, s.I[0], s.I[1]})
+            if (e == hipSuccess && s.abytes > fbytes)
+                e = hipMemsetAsync(reinterpret_cast<char*>(a) + fbytes, 0, s.abytes - fbytes, b->stream);
     } else if (e == hipSuccess) {
         if ((e = dev_alloc(b, sbytes, &s.state[0])) == hipSuccess && (e = dev_alloc(b, sbytes, &s.state[1])) == hipSuccess)
             e = dev_alloc(b, s.cbytes, (void**)&s.cell_inv);

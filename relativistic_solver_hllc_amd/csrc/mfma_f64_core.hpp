@@ -9,6 +9,16 @@ namespace rsh {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// Global-address-space views. A pointer the kernel reads from a descriptor in
+// memory (the grouped kernels' problem table) is generic, and loads through it
+// are "flat" loads, which also count on the LDS counter: every wait for an LDS
+// read would then wait for the global loads in flight as well.
+typedef const __attribute__((address_space(1))) double* gcdouble_p;
+typedef __attribute__((address_space(1))) double* gdouble_p;
+typedef const __attribute__((address_space(1))) v2d* gcv2d_p;
+__device__ __forceinline__ gcdouble_p as_global(const double* p) { return (gcdouble_p)p; }
+__device__ __forceinline__ gdouble_p as_global(double* p) { return (gdouble_p)p; }
+
 template <int RB, int W, int SC>
 struct F64TileShape {
     static constexpr int FRAG = 64;                   // doubles per fragment
@@ -36,9 +46,9 @@ __device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ i
     const bool cvalid = cell < n;
 
     // Panel of this pass: steps * RB fragments, stage t = units [t*UNITS, (t+1)*UNITS).
-    const v2d* psrc = reinterpret_cast<const v2d*>(panel_pass) + tid;
+    const gcv2d_p psrc = (gcv2d_p)as_global(panel_pass) + tid;
     // Face pointer of (j = jr, cell); one j-step advances 4 directions.
-    const double* xp = illum + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
+    const gcdouble_p xp = as_global(illum) + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
     const int64_t xstep = 4 * n * FACES;
 
 #pragma unroll
@@ -62,10 +72,10 @@ __device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ i
     _Pragma("unroll") for (int s = 0; s < SC; s++) {                          \
         const int step = (t) * SC + s;                                        \
         if (cvalid && 4 * step + jr < m) {                                    \
-            const double* p = xp + (int64_t)step * xstep;                     \
+            const gcdouble_p p = xp + (int64_t)step * xstep;                  \
             if constexpr (FACES == 4) {                                       \
-                const v2d a = reinterpret_cast<const v2d*>(p)[0];     \
-                const v2d b = reinterpret_cast<const v2d*>(p)[1];     \
+                const v2d a = ((gcv2d_p)p)[0];                                \
+                const v2d b = ((gcv2d_p)p)[1];                                \
                 xr[s][0] = a.x; xr[s][1 % FACES] = a.y;                       \
                 xr[s][2 % FACES] = b.x; xr[s][3 % FACES] = b.y;               \
             } else {                                                          \

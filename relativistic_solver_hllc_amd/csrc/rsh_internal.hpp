@@ -32,12 +32,13 @@ constexpr int kPanelStepAlign = 4;      // j-steps padded to a multiple of this
 constexpr int kMaxRowBlocksPerPass = 25; // 400 output directions per pass
 // The grouped iteration kernel's fused epilogue (B and the previous state read
 // per output) is latency-bound at one workgroup per CU; passes of at most 13 row
-// blocks (208 rows; 166 VGPRs at 4 waves x 2 j-steps per LDS stage) let three
-// workgroups share a CU, so one's epilogue overlaps the others' MFMA work, for
-// 4 % padding at M = 200, 400 and 800 and a re-read of the state per pass.
+// blocks (208 rows, at 4 waves x 2 j-steps per LDS stage) let two workgroups
+// share a CU, so one's epilogue overlaps the other's MFMA work, for 4 %
+// padding at M = 200, 400 and 800 and a re-read of the state per pass.
 constexpr int kGroupedF64MaxRb = 13;
 constexpr int kGroupedF64Waves = 4;
 constexpr int kGroupedF64ChunkSteps = 2;
+constexpr int64_t kGroupedF64MaxCells = int64_t(1) << 24;  // epilogue offsets: 16 rows x n x 8 B < 2^31
 
 struct F64Plan {
     int rb_per_pass;   // row blocks per workgroup pass (template value)
