@@ -43,7 +43,7 @@ struct Slot {
     float* cell_inv = nullptr;
     double* B = nullptr;
     size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I (M rows), state, cell_inv
-    size_t abytes = 0;                          // allocated size of B / I (FP64: padded rows)
+    size_t abytes = 0;                          // allocated size of B and (FP64) I: padded rows
     F64Plan plan{};
     SplitPlan splan{};
     bool res = false;  // split form on the resident-panel grouped kernel
@@ -300,16 +300,18 @@ int capture(rsh_batch b) {
     return RSH_OK;
 }
 
-// Rows allocated per FP64 state / source array: the iteration plan's padded
-// row count (whole row blocks). The grouped kernel reads and writes the padding
-// rows (zero, so their residual term is 0 / 0 and ignored) instead of
-// branching per row.
-int64_t f64_rows(int m) { return (int64_t)plan_f64(m, m, kGroupedF64MaxRb).rb_total * 16; }
+// Rows allocated per source array (and FP64 state array): whole row blocks,
+// for FP64 the iteration plan's padded row count. The grouped kernels read (and
+// the FP64 one writes) the padding rows, which are zero, so that their residual
+// term is 0 / 0 and ignored, instead of branching per row.
+int64_t padded_rows(int form, int m) {
+    return form == RSH_FORM_FP64 ? (int64_t)plan_f64(m, m, kGroupedF64MaxRb).rb_total * 16 : (m + 15) / 16 * 16;
+}
 
 int64_t footprint(int form, int64_t n, int m) {
-    if (form == RSH_FORM_FP64) return 3 * f64_rows(m) * n * 8;  // I ping-pong + B, padded rows
-    if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)
-        return 2 * split_state_units(form, 1, n, m) * 16 + (int64_t)m * n * 8 + n * 4;  // state ping-pong + B + scales
+    if (form == RSH_FORM_FP64) return 3 * padded_rows(form, m) * n * 8;  // I ping-pong + B, padded rows
+    if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)  // state ping-pong + B + scales
+        return 2 * split_state_units(form, 1, n, m) * 16 + padded_rows(form, m) * n * 8 + n * 4;
     return -1;
 }
 
@@ -432,14 +434,15 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
     const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
     s.fbytes = fbytes;
-    s.abytes = form == RSH_FORM_FP64 ? (size_t)f64_rows(ds->m) * n_cells * sizeof(double) : fbytes;
+    s.abytes = (size_t)padded_rows(form, ds->m) * n_cells * sizeof(double);
     s.sbytes = sbytes;
     s.cbytes = form == RSH_FORM_FP64 ? 0 : (size_t)n_cells * sizeof(float);
     hipError_t e = dev_alloc(b, s.abytes, (void**)&s.B);
+    if (e == hipSuccess && s.abytes > fbytes)  // zero padding rows of B (read by the iteration kernels)
+        e = hipMemsetAsync(reinterpret_cast<char*>(s.B) + fbytes, 0, s.abytes - fbytes, b->stream);
     if (e == hipSuccess && form == RSH_FORM_FP64) {
         if ((e = dev_alloc(b, s.abytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, s.abytes, (void**)&s.I[1]);
-        // zero padding rows: the iteration kernel reads them and writes zeros back
-        for (double* a : {s.B, s.I[0], s.I[1]})
+        for (double* a : {s.I[0], s.I[1]})  // zero padding rows (read, and written back as zeros)
             if (e == hipSuccess && s.abytes > fbytes)
                 e = hipMemsetAsync(reinterpret_cast<char*>(a) + fbytes, 0, s.abytes - fbytes, b->stream);
     } else if (e == hipSuccess) {

@@ -437,6 +437,7 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
             int zoff;
             asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
             const float* __restrict__ rs = rscale + 4 * g + zoff;
+            const double oma = P.one_minus_albedo, alb = P.albedo;  // in registers across the stores
 #pragma unroll
             for (int q = 0; q < kResTiles; q++) {
                 const int tl = wave * kResTiles + q;  // tile within the chunk
@@ -454,6 +455,17 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
                     const int voff = (tl * (int)tile_units + vlane) * 16 + 8 * (g & 1);
                     const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
                         const_cast<double*>(P.B) + (int64_t)rbg * 16 * n, (short)0, (int)(16 * n * 8), kBufWord3);
+                    // B rows exist up to the row block's end (the source array
+                    // holds whole row blocks, padding rows zero), so all loads of
+                    // the row block are issued together, with no per-row branch.
+                    // A padding row (k >= M) has zero panel rows, zero B and zero
+                    // old state: nv = 0, parts 0, residual term 0 / 0 = NaN,
+                    // which fmax drops.
+                    double bv[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        bv[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                               bsrc, (int)(4 * g * n * 8) + cbyte, (int)(r * n * 8), 0));
                     T oldp[NT][4], newp[NT][4];
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
@@ -466,27 +478,17 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
                     }
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int k = rbg * 16 + 4 * g + r;
-                        if (k < P.m) {
-                            const double bv = __builtin_bit_cast(
-                                double, __builtin_amdgcn_raw_buffer_load_b64(bsrc, (int)(4 * g * n * 8) + cbyte,
-                                                                             (int)(r * n * 8), 0));
-                            const double S = ((double)accm[i][q][r] + (double)accx[i][q][r]) *
-                                             (double)rs[i * 16 + r] * ci;
-                            const double nv = P.one_minus_albedo * bv + P.albedo * S;
-                            float of = 0.0f;
+                        const double S = ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
+                        const double nv = oma * bv[r] + alb * S;
+                        float of = 0.0f;
 #pragma unroll
-                            for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
-                            const double old = (double)of * ci;
-                            res = fmax(res, fabs(old - nv) / nv);
-                            T qp[NT];
-                            split_parts<FMT, NT>((float)(nv * cscale), qp);
+                        for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
+                        const double old = (double)of * ci;
+                        res = fmax(res, fabs(old - nv) / nv);
+                        T qp[NT];
+                        split_parts<FMT, NT>((float)(nv * cscale), qp);
 #pragma unroll
-                            for (int t = 0; t < NT; t++) newp[t][r] = qp[t];
-                        } else {
-#pragma unroll
-                            for (int t = 0; t < NT; t++) newp[t][r] = (T)0.0f;
-                        }
+                        for (int t = 0; t < NT; t++) newp[t][r] = qp[t];
                     }
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
