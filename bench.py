@@ -289,6 +289,13 @@ def sweep_section(D, rs, edge=32, m=400, steps=5, cpu=True):
     ds.close()
     sp.close()
     gbs = byts / (t * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(REPO, "profiles", f"pmc_sweep_{n}x{m}.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+        if pmc.get("n_cells") == n and pmc.get("n_dirs") == m:
+            traffic = pmc.get("hbm_bytes_per_launch")
     log(f"[bench] sweep {n} cells x {m} dirs: {t:.3f} ms ({n * m / (t * 1e-3):.3e} cell-dir/s, {gbs:.0f} GB/s), "
         f"levels {levels.min()}-{levels.max()}, max rel err vs oracle {err:.2e}; iteration {t_iter:.3f} ms"
         + (f"; cpu {cpu_res['seconds']:.2f} s" if cpu_res else ""))
@@ -299,7 +306,7 @@ def sweep_section(D, rs, edge=32, m=400, steps=5, cpu=True):
             "levels_mean": float(levels.mean()), "levels_max": int(levels.max()),
             "prepare_s": prep_s, "upload_s": upload_s, "records": records,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": byts},
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": byts},
             "iteration_ms": t_iter, "iteration": "sweep + FP64 scattering integral (mfma_f64) of the new faces",
             "residual": norm,
             "parity": {"directions_checked": len(sub), "max_rel_err_vs_oracle": err, "tolerance": 1e-12,
