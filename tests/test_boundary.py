@@ -140,3 +140,24 @@ def test_compare_files_matches_definition(tmp_path):
         assert total == pytest.approx(sum(d), rel=1e-15)
     with pytest.raises(ValueError):
         rs.compare_files(f1, f2, "int64")
+
+
+
+@pytest.mark.parametrize("m,rows", [(1, 16), (16, 16), (26, 32), (200, 208), (208, 208), (209, 224), (400, 416),
+                                    (800, 832)])
+def test_batch_footprint_counts_padded_rows(m, rows):
+    """rsh_batch_footprint (host arithmetic, no device) counts whole row blocks.
+    FP64 slots hold I twice and B with the iteration plan's rows: passes of at
+    most 13 row blocks of 16 (M = 209 -> 2 passes of 7 row blocks = 224 rows)."""
+    n = 1000
+    assert rs.Batch.footprint("fp64", n, m) == 3 * rows * n * 8
+
+
+@pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
+def test_split_footprint_pads_source_rows(form):
+    """Split slots hold B padded to a multiple of 16 rows: M = 17 and M = 32 need
+    the same memory (one 32-direction K-step of state, 32 rows of B), M = 16 one
+    row block of B less."""
+    n = 1000
+    assert rs.Batch.footprint(form, n, 17) == rs.Batch.footprint(form, n, 32)
+    assert rs.Batch.footprint(form, n, 17) - rs.Batch.footprint(form, n, 16) == 16 * n * 8

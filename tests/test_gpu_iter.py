@@ -306,3 +306,18 @@ def test_split_resident_kernel_equals_streaming(gpu, form):
             add_form(b, probs[3], form)        # M = 161: no specialised resident plan
     for p in probs:
         p["ds"].close()
+
+
+def test_fp64_batch_cell_limit(gpu):
+    """FP64 batch problems hold fewer than 2^24 cells (the iteration kernel's
+    32-bit buffer offsets span one row block of 16 rows); larger ones are refused
+    before any allocation, and the batch stays usable."""
+    m = 26
+    xyz, w, F = rs.fibonacci_directions(m)
+    with rs.DirectionSet(xyz, w, F, normalize=True) as ds, rs.Batch(2) as b:
+        with pytest.raises(rs.N.RshError, match="2\\^24"):
+            b.add_seeded(ds, 1 << 24, 1, 0.5)
+        s = b.add_seeded(ds, 5000, 2, 0.5)
+        b.run(2)
+        r, it = b.residuals()
+        assert it[s] == 2 and np.isfinite(r[s])
