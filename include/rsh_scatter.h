@@ -253,8 +253,10 @@ int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
  * (Illum_async.cpp:663-1060); here problems share persistent grouped launches
  * (one per row-block class) captured as a hipGraph, the state stays resident,
  * and problems join and leave between runs (continuous batching).
- *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM)
- *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot,
+ *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM;
+ *                        slots hold whole row blocks of 16 directions)
+ *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot
+ *                        (FP64 form: n_cells < 2^24, else RSH_E_UNSUPPORTED),
  *                        ordered after all work submitted to the null stream
  *                        (arrays written on other streams must be complete);
  *                        ds holds the directions and phase matrix; form selects

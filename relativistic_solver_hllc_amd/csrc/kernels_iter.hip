@@ -37,9 +37,12 @@ __global__ __launch_bounds__(256) void batch_zero_kernel(BatchProblem* __restric
 // a buffer descriptor per row block: one per-lane 32-bit offset (cell and jr)
 // and a uniform offset per row, so a load costs no VGPRs beyond its result.
 // The loads (B and the previous state) of row block rb + kEpiAhead are issued,
-// in program order, before the stores of row block rb. Offsets stay below
-// 2^31 for n < 2^24 cells (checked on the host). Returns the lane's max
-// relative change.
+// in program order, before the stores of row block rb: with kEpiAhead = 0 the
+// 8 loads of a row block go out together (13 round trips per item at RB = 13,
+// not 52); looking 1 or 2 row blocks further ahead needs more than the 168
+// VGPRs of 3 waves per SIMD and measured slower at 2. Offsets stay below 2^31
+// for n < 2^24 cells (checked on the host). Returns the lane's max relative
+// change.
 constexpr int kEpiAhead = 0;
 constexpr int kBufWord3 = 0x00020000;  // raw (untyped) gfx9 buffer access, 32-bit format
 typedef unsigned int v2u __attribute__((ext_vector_type(2)));
@@ -70,7 +73,7 @@ __device__ __forceinline__ double iterate_epilogue(const double* bsrc, const dou
     double res = 0.0;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
-        if (rb + kEpiAhead < RB) load(rb + kEpiAhead);  // kEpiAhead = 0: this row block's own loads
+        if (rb + kEpiAhead < RB) load(rb + kEpiAhead);
         const __amdgpu_buffer_rsrc_t od = rsrc(odst, rb);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
