@@ -29,9 +29,11 @@ namespace rsh {
 
 typedef double v2d_m __attribute__((ext_vector_type(2)));
 
-// 128-thread workgroups and four rows in flight per thread measured fastest
-// (83 % of the 8 TB/s HBM peak at 1.57 M cells x 400 directions, against 88 %
-// for a pure streaming read of the same bytes).
+// 128-thread workgroups and four rows in flight per thread: 72-74 % of the
+// 8 TB/s HBM peak at 1.57 M cells x 400 directions (a pure streaming read of
+// the same bytes: 88 %). Blocks of 64-1024 threads with 2-16 rows in flight
+// measured 69-79 %, with the same configuration differing by up to 10 % from
+// one process to the next, so none is reliably faster.
 constexpr int kMomentBlock = 128;
 constexpr int kMomentPipe = 4;
 
