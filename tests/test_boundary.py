@@ -161,3 +161,22 @@ def test_split_footprint_pads_source_rows(form):
     n = 1000
     assert rs.Batch.footprint(form, n, 17) == rs.Batch.footprint(form, n, 32)
     assert rs.Batch.footprint(form, n, 17) - rs.Batch.footprint(form, n, 16) == 16 * n * 8
+
+
+def test_import_fails_loudly_without_the_library(tmp_path):
+    """A copy of the package without librsh_scatter.so refuses to load (no Python
+    or CPU fallback): importing its scatter module raises ImportError naming the
+    build command."""
+    import shutil
+    import subprocess
+    import sys
+    src = os.path.join(REPO, "relativistic_solver_hllc_amd")
+    dst = tmp_path / "relativistic_solver_hllc_amd"
+    dst.mkdir()
+    for f in os.listdir(src):
+        if f.endswith(".py"):
+            shutil.copy(os.path.join(src, f), dst / f)
+    proc = subprocess.run([sys.executable, "-c", "import relativistic_solver_hllc_amd.scatter"], cwd=tmp_path,
+                          capture_output=True, text=True, timeout=120)
+    assert proc.returncode != 0
+    assert "ImportError" in proc.stderr and "is not built" in proc.stderr
