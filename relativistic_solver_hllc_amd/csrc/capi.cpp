@@ -718,7 +718,10 @@ int rsh_create(const rsh_problem_desc* d, rsh_problem* out) {
         (e = hipMalloc(&p->d_S, std::max<size_t>(s_bytes, 8))) != hipSuccess)
         return cleanup(hip_fail(e, "hipMalloc(problem buffers)"));
     for (int i = 0; i < RSH_STREAM_COUNT; i++) {
-        // Default priority on purpose (see DESIGN notes in the repository).
+        // Default priority on purpose: the reference creates these three streams with
+        // high / mid / low priorities (cuda_solve.cu:60-66); work on streams of
+        // different priorities has been seen to hang on this platform, and the
+        // ordering the callers rely on comes from events, not priorities.
         if ((e = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking)) != hipSuccess ||
             (e = hipEventCreateWithFlags(&p->upload_ev[i], hipEventDisableTiming)) != hipSuccess ||
             (e = hipEventCreate(&p->t0[i])) != hipSuccess || (e = hipEventCreate(&p->t1[i])) != hipSuccess)
