@@ -142,15 +142,15 @@ struct PassDyn {
 __device__ inline void load_dyn(const PassIn& x, const double* __restrict__ Sk, const double* fv, PassDyn& y) {
     typedef double v2d __attribute__((ext_vector_type(2)));
     // Unconditional loads at always-valid addresses (no branch around a load,
-    // so the wait counters stay exact): entries a cell does not need are read
-    // from its own row and ignored.
-    const int64_t own = (int64_t)x.p.cell * 4;
+    // so the wait counters stay exact): a face that needs no upstream value
+    // reads the direction's first entry, the same address in every lane, so
+    // that such lanes add no line requests to the load (the value is ignored).
     y.S = Sk[x.p.cell];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const int p = x.p.pr[f];
         const bool up = ((x.p.meta >> f) & 1) && p >= 0;
-        y.fin[f] = fv[up ? (int64_t)p : own + f];  // upstream cell's own entry
+        y.fin[f] = fv[up ? (int64_t)p : 0];  // upstream cell's own entry
     }
 }
 
