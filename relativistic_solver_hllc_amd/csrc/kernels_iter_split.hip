@@ -59,12 +59,30 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
             const double cinv = (double)P.cell_inv[cell];
             const double cscale = 1.0 / cinv;  // power of two: exact
             const unsigned char* in_bytes = reinterpret_cast<const unsigned char*>(st_in);
+            // descriptor fields in registers across the stores; global-space source
+            const double oma = P.one_minus_albedo, alb = P.albedo;
+            const int m = P.m;
+            const int64_t n = P.n;
+            const __attribute__((address_space(1))) double* bsrc = (const __attribute__((address_space(1))) double*)P.B;
+            const __attribute__((address_space(1))) float* rinv = (const __attribute__((address_space(1))) float*)P.row_inv;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 const int rbg = item.pass * RB + rb;
-                if (rbg * 16 >= P.m) continue;  // padding row block: no rows, no state
+                if (rbg * 16 >= m) continue;  // padding row block: no rows, no state
                 const int s = rbg >> 1;
                 const int vlane = (2 * (rbg & 1) + (g >> 1)) * 16 + cl;
+                // The source array holds whole row blocks (padding rows zero), so the
+                // row block's B loads go out together with no per-row branch; a
+                // padding row (k >= M) gets row scale 0, hence S = 0, nv = 0, parts
+                // 0 and a residual term 0 / 0 = NaN, which fmax drops.
+                double bv[4];
+                float ri[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int k = rbg * 16 + 4 * g + r;
+                    bv[r] = bsrc[(int64_t)k * n + cell];
+                    ri[r] = k < m ? rinv[k] : 0.0f;
+                }
                 T oldp[NT][4], newp[NT][4];
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
@@ -77,24 +95,17 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 }
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int k = rbg * 16 + 4 * g + r;
-                    if (k < P.m) {
-                        const double S =
-                            ((double)accm[rb][r] + (double)accx[rb][r]) * (double)P.row_inv[k] * cinv;
-                        const double nv = P.one_minus_albedo * P.B[(int64_t)k * P.n + cell] + P.albedo * S;
-                        float of = 0.0f;
+                    const double S = ((double)accm[rb][r] + (double)accx[rb][r]) * (double)ri[r] * cinv;
+                    const double nv = oma * bv[r] + alb * S;
+                    float of = 0.0f;
 #pragma unroll
-                        for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
-                        const double old = (double)of * cinv;
-                        res = fmax(res, fabs(old - nv) / nv);
-                        T q[NT];
-                        split_parts<FMT, NT>((float)(nv * cscale), q);
+                    for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
+                    const double old = (double)of * cinv;
+                    res = fmax(res, fabs(old - nv) / nv);
+                    T q[NT];
+                    split_parts<FMT, NT>((float)(nv * cscale), q);
 #pragma unroll
-                        for (int t = 0; t < NT; t++) newp[t][r] = q[t];
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NT; t++) newp[t][r] = (T)0.0f;
-                    }
+                    for (int t = 0; t < NT; t++) newp[t][r] = q[t];
                 }
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
