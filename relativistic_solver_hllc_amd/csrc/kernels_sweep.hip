@@ -10,8 +10,16 @@
 // walks its cells level by level: a cell's level is one more than the levels of
 // the cells it reads from (the neighbours across its in faces) and of the cells
 // that write a face it also writes, so all cells of a level are independent and
-// run in parallel, with a workgroup barrier between levels. The face values a
-// level writes reach the next level through global memory inside one CU.
+// run in parallel. Every pass (at most one cell per thread) ends at a barrier
+// that orders only LDS: a pass also writes its face values into an LDS ring of
+// the last four passes, and a cell whose upstream neighbour was swept one to
+// three passes earlier reads it there (the host marks each such in face with
+// the pass distance and the neighbour's index in its pass); older upstream
+// values come from global memory, where they are complete by then: a wave's
+// stores of pass q are done once it has waited for a load issued after them,
+// which it does in pass q + 2, before the barrier that ends pass q + 2, and
+// such a value is read in pass q + 4 or later. The full fence (store
+// completion) only precedes the residual pass.
 //
 // Face values. The reference keeps three node values per face and only ever
 // uses their mean ((c0 + c1 + c2) / 3, both when a downstream cell reads its in
@@ -37,11 +45,11 @@
 // first cell (identical values), so no branch surrounds a load or a store and
 // the wait counters stay exact.
 //
-// Measured limit (196,608 cells x 400 directions): ~7 ms per sweep, of which
+// Measured (196,608 cells x 400 directions): 6.4 ms per sweep (6.9 ms with
+// global memory and a store-completing barrier between levels), of which
 // ~1.2 ms the residual pass; the FP64 arithmetic class barely matters (pure
 // transport, class 3, is 4 % faster than class 1), L2 misses 61 % of requests
-// (the per-cell reads of S and the level's neighbour entries are scattered), so
-// the sweep is bound by scattered-line memory traffic, ~3 TB/s of lines.
+// (the per-cell reads of S and the level's neighbour entries are scattered).
 #include "rsh_internal.hpp"
 
 namespace rsh {
@@ -114,6 +122,8 @@ __device__ inline void load_pass(const SweepDirDev& d, int pi, PassIn& x) {
     x.p.meta = pp->meta;
 #pragma unroll
     for (int f = 0; f < 4; f++) x.p.pr[f] = pp->pr[f];
+    x.p.ring[0] = pp->ring[0];
+    x.p.ring[1] = pp->ring[1];
     const auto* sp = gptr(reinterpret_cast<const v2d*>(d.s9 + (int64_t)i * kSweepRecStride));
 #pragma unroll
     for (int q = 0; q < kSweepRecStride / 2; q++) {
@@ -139,19 +149,42 @@ struct PassDyn {
     double fin[4];
 };
 
-__device__ inline void load_dyn(const PassIn& x, const double* __restrict__ Sk, const double* fv, PassDyn& y) {
+// LDS ring of the last kSweepRing passes' face values: [slot][face][index].
+typedef double SweepRing[kSweepRing][4][kSweepBlock];
+
+// rp: index of the reading pass (ring slot of an upstream pass = its index mod
+// kSweepRing).
+__device__ inline void load_dyn(const PassIn& x, const double* __restrict__ Sk, const double* fv, PassDyn& y,
+                                const SweepRing& ring, int rp) {
     typedef double v2d __attribute__((ext_vector_type(2)));
     // Unconditional loads at always-valid addresses (no branch around a load,
-    // so the wait counters stay exact): a face that needs no upstream value
-    // reads the direction's first entry, the same address in every lane, so
-    // that such lanes add no line requests to the load (the value is ignored).
+    // so the wait counters stay exact): a face whose value comes from the ring,
+    // or that needs none, reads the direction's first entry in global memory,
+    // the same address in every lane, so that such lanes add no line requests
+    // (the value is ignored); the ring read of a face without a ring code reads
+    // slot 0, face 0, index 0 and is ignored too.
     y.S = Sk[x.p.cell];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const int p = x.p.pr[f];
         const bool up = ((x.p.meta >> f) & 1) && p >= 0;
-        y.fin[f] = fv[up ? (int64_t)p : 0];  // upstream cell's own entry
+        const uint32_t code = (x.p.ring[f >> 1] >> (16 * (f & 1))) & 0xffffu;
+        const bool in_ring = up && code != 0;
+        const double g = fv[up && !in_ring ? (int64_t)p : 0];  // upstream cell's own entry
+        const int slot = in_ring ? (rp - (int)(code >> 9)) & (kSweepRing - 1) : 0;
+        const double l = ring[slot][in_ring ? (p & 3) : 0][in_ring ? (int)(code & 511u) : 0];
+        y.fin[f] = in_ring ? l : g;
     }
+}
+
+// Barrier that orders only LDS accesses: global stores may stay in flight
+// across it (an upstream value read from global memory was stored at least
+// kSweepRingDelta + 1 passes earlier, and every wave has waited for a load
+// issued after its stores of pass q - 2 before it reaches the barrier of pass q).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <int CLS>
@@ -166,6 +199,8 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
     double* fv = new_illum + (int64_t)k * n * 4;
     const double* __restrict__ Sk = S + (int64_t)k * ld;
     const double* __restrict__ ok = old_illum ? old_illum + (int64_t)k * n * 4 : nullptr;
+    extern __shared__ double ring_mem[];
+    SweepRing& ring = *reinterpret_cast<SweepRing*>(ring_mem);
     double loc = -1;
     // One pass: compute from (cur, dc) and fetch the next pass into (nxt, dn).
     // The loop below is unrolled by two with the buffers swapping roles, so no
@@ -210,7 +245,7 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
                 own[3] = f == 3 ? v : own[3];
             }
         }
-        if (!last) load_dyn(nxt, Sk, fv, dn);  // same level: its upstream values are final
+        if (!last) load_dyn(nxt, Sk, fv, dn, ring, pi + 1);  // same level: its upstream values are final
         {
             v2d* dst = reinterpret_cast<v2d*>(fv + cell * 4);
             dst[0] = v2d{own[0], own[1]};
@@ -219,17 +254,21 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
             for (int f = 0; f < 4; f++) {
                 if ((meta >> (4 + f)) & 1) fv[cur.p.pr[f]] = own[f];  // out on both sides, this cell later
             }
+            // this pass's values into its ring slot (threads past the pass's end
+            // write their own, never-read indices)
+#pragma unroll
+            for (int f = 0; f < 4; f++) ring[pi & (kSweepRing - 1)][f][threadIdx.x] = own[f];
         }
-        if (last) {
-            __syncthreads();
-            load_dyn(nxt, Sk, fv, dn);  // next level: after the barrier
-        }
+        // every pass ends at a barrier: no wave runs more than a pass ahead, so
+        // a ring slot is rewritten only after every read of its previous pass
+        lds_barrier();
+        if (last) load_dyn(nxt, Sk, fv, dn, ring, pi + 1);  // next level: after the barrier
     };
     PassIn pa, pb;
     PassDyn da, db;
     if (d.passes > 0) {
         load_pass(d, 0, pa);
-        load_dyn(pa, Sk, fv, da);
+        load_dyn(pa, Sk, fv, da, ring, 0);  // level 0: no upstream values
     }
     for (int pi = 0; pi < d.passes; pi += 2) {
         step(pa, da, pb, db, pi);
@@ -257,8 +296,15 @@ hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, co
                         hipStream_t st) {
     if (m <= 0 || n <= 0) return hipSuccess;
     const dim3 grid((unsigned)m), block(kSweepBlock);
+    constexpr int kRingBytes = (int)sizeof(SweepRing);
 #define RSH_SWEEP(C)                                                                                             \
-    hipLaunchKernelGGL(sweep_kernel<C>, grid, block, 0, st, d_dirs, n, d_nodes, d_S, ld, d_old, d_new, d_norm)
+    do {                                                                                                         \
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sweep_kernel<C>),                      \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kRingBytes);             \
+        if (e != hipSuccess) return e;                                                                           \
+        hipLaunchKernelGGL(sweep_kernel<C>, grid, block, kRingBytes, st, d_dirs, n, d_nodes, d_S, ld, d_old,     \
+                           d_new, d_norm);                                                                       \
+    } while (0)
     switch (cls) {
         case 0: RSH_SWEEP(0); break;
         case 1: RSH_SWEEP(1); break;

@@ -91,6 +91,12 @@ hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1
 // stored per position in level order, so that it can be fetched a pass ahead.
 constexpr int kSweepBlock = 512;
 constexpr int kSweepRecStride = 10;  // record slots per position (9 used; 80 bytes = five 16-byte loads)
+// The face values of the last kSweepRing passes stay in an LDS ring (one slot
+// of kSweepBlock x 4 doubles per pass); an upstream value written at most
+// kSweepRingDelta passes before the reading pass is read from there, older
+// ones from global memory (complete by then: see kernels_sweep.hip).
+constexpr int kSweepRing = 4;
+constexpr int kSweepRingDelta = 3;
 struct SweepPos {
     int32_t cell;
     // bits 0-3: in faces; 4-7: faces out on both sides whose neighbour came
@@ -98,7 +104,10 @@ struct SweepPos {
     // of record slot q (q = 3 r + j: node j of the r-th out face)
     uint32_t meta;
     int32_t pr[4];  // pairs of the cell
-    int32_t pad[2];
+    // ring code of in face f in bits 16 (f % 2) of ring[f / 2]: 0 = read the
+    // upstream entry from global memory, else (passes back << 9) | index of the
+    // upstream cell in its pass
+    uint32_t ring[2];
 };
 struct SweepDirDev {
     const SweepPos* pos;   // [n] by position in level order

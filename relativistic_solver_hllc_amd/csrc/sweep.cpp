@@ -155,7 +155,7 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
         uint32_t meta = (uint32_t)st;
         SweepPos& P = posv[(size_t)i];
         P.cell = c;
-        P.pad[0] = P.pad[1] = 0;
+        P.ring[0] = P.ring[1] = 0;
         int q = 0;
         for (int f = 0; f < 4; f++) {
             const int32_t p = s->pairs[(size_t)(c * 4 + f)];
@@ -182,6 +182,28 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
             pass.push_back(cnt | (last ? (int32_t)0x80000000u : 0));
         }
     const int32_t passes = (int32_t)(pass.size() / 2);
+    // Ring codes: an in face whose upstream cell was swept 1..kSweepRingDelta
+    // passes earlier reads that cell's values from the LDS ring.
+    {
+        std::vector<int32_t> pass_of((size_t)n), where((size_t)n);
+        for (int32_t pi = 0; pi < passes; pi++) {
+            const int32_t b0 = pass[(size_t)(2 * pi)], cnt = pass[(size_t)(2 * pi + 1)] & 0x7fffffff;
+            for (int32_t j = 0; j < cnt; j++) pass_of[(size_t)(b0 + j)] = pi;
+        }
+        for (int64_t i = 0; i < n; i++) where[(size_t)cells[(size_t)i]] = (int32_t)i;
+        for (int64_t i = 0; i < n; i++) {
+            SweepPos& P = posv[(size_t)i];
+            for (int f = 0; f < 4; f++) {
+                const int32_t p = P.pr[f];
+                if (p < 0 || !((P.meta >> f) & 1)) continue;
+                const int32_t up = where[(size_t)(p / 4)];
+                const int32_t delta = pass_of[(size_t)i] - pass_of[(size_t)up];
+                if (delta < 1 || delta > kSweepRingDelta) continue;
+                const uint32_t code = ((uint32_t)delta << 9) | (uint32_t)(up - pass[(size_t)(2 * pass_of[(size_t)up])]);
+                P.ring[f / 2] |= code << (16 * (f % 2));
+            }
+        }
+    }
 
     const size_t b_pos = align256((size_t)n * sizeof(SweepPos)), b_s9 = align256((size_t)n * kSweepRecStride * 8),
                  b_pass = align256(pass.size() * 4);
