@@ -403,6 +403,18 @@ int rsh_copy_illum_sync(rsh_problem p, const double* host, int64_t count);
  * offset "+start" (cuda_solve.cu:107) is corrected to "+start*N". */
 int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* host_out);
 
+/* Upload of the whole host illum [M][n_cells][faces] and the scattering
+ * integral for [k0, k1), pipelined by cell range: the faces go up in `chunks`
+ * cell ranges (2-D copies on stream RSH_STREAM_SCATTERING_1), each range's S is
+ * computed as soon as it has landed (RSH_STREAM_SCATTERING_2) and, when
+ * host_out is not NULL, copied back (RSH_STREAM_PARAMS) while the next range
+ * uploads. Returns when everything is done; the device illum then holds the
+ * whole upload, as after rsh_copy_illum_sync. The synchronous sequence of
+ * CalculateIntScattering (cuda/cuda_solve.cu:153-170: upload, compute, copy
+ * back) with the three steps overlapped; chunks = 1 is that sequence. */
+int rsh_scatter_from_host(rsh_problem p, const double* host_illum, int32_t k0, int32_t k1, double* host_out,
+                          int32_t chunks);
+
 /* Waits for a stream id; stream < 0 waits for the whole device.
  * Replaces CudaSyncStream and CudaWait (cuda/cuda_solve.cu:37-45). */
 int rsh_sync(rsh_problem p, int32_t stream);

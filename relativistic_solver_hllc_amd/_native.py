@@ -139,6 +139,7 @@ PROTOTYPES = {
     "rsh_upload_illum": (C.c_int, [vp, vp, i64, i64, i64, i32]),
     "rsh_copy_illum_sync": (C.c_int, [vp, vp, i64]),
     "rsh_scatter": (C.c_int, [vp, i32, i32, i32, vp]),
+    "rsh_scatter_from_host": (C.c_int, [vp, vp, i32, i32, vp, i32]),
     "rsh_sync": (C.c_int, [vp, i32]),
     "rsh_problem_buffers": (C.c_int, [vp, P_vp, P_vp]),
     "rsh_problem_stream": (C.c_int, [vp, i32, P_vp]),

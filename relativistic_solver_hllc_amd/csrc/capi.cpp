@@ -131,16 +131,20 @@ void set_tuned_tile(int device, int rb, rsh_tile t) {
 
 namespace {
 
-int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int k0, int k1, double* d_out,
-                int64_t ld, hipStream_t st, int kernel, const rsh_tile* tile) {
+// n cells of the input, whose direction rows are ldx cells apart (ldx = n for
+// a whole array, larger for a cell range of one).
+int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int64_t ldx, int k0, int k1,
+                double* d_out, int64_t ld, hipStream_t st, int kernel, const rsh_tile* tile) {
     if (!ds) return fail(RSH_E_ARG, "null direction set");
     if (faces != 1 && faces != 4) return fail(RSH_E_UNSUPPORTED, "faces must be 1 or 4");
     if (n < 0 || k0 < 0 || k1 > ds->m || k0 > k1) return fail(RSH_E_ARG, "invalid cell count or direction range");
     if (n == 0 || k0 == k1) return RSH_OK;
     if (!d_illum || !d_out) return fail(RSH_E_ARG, "null device pointer");
     if (ld < n) return fail(RSH_E_ARG, "ld_out smaller than n_cells");
+    if (ldx < n) return fail(RSH_E_ARG, "input row stride smaller than n_cells");
     RSH_HIP_TRY(hipSetDevice(ds->device));
     if (kernel == RSH_F64_DIRECT) {
+        if (ldx != n) return fail(RSH_E_UNSUPPORTED, "the reference-order kernel reads whole arrays only");
         RSH_HIP_TRY(launch_direct_f64(d_illum, faces, n, ds->m, ds->d_dirs, ds->d_w, ds->full_area, ds->d_P,
                                       ds->ref_order() ? 1 : 0, k0, k1, d_out, ld, st));
         return RSH_OK;
@@ -153,8 +157,8 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int 
     const PanelEntry* pe = nullptr;
     int rc = get_panel(ds, k0, k1, faces, RSH_FORM_FP64, st, &pe);
     if (rc) return rc;
-    RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ds->m, static_cast<const double*>(pe->d_panel), pe->plan, k1 - k0,
-                                d_out, ld, t.waves, t.chunk_steps, st));
+    RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ldx, ds->m, static_cast<const double*>(pe->d_panel), pe->plan,
+                                k1 - k0, d_out, ld, t.waves, t.chunk_steps, st));
     return RSH_OK;
 }
 
@@ -419,7 +423,7 @@ int rsh_dirset_phase_matrix(rsh_dirset ds, int32_t k0, int32_t k1, double* host_
 int rsh_scatter_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells, int32_t k0,
                         int32_t k1, double* d_out, int64_t ld_out, void* stream, int32_t kernel,
                         const rsh_tile* tile) {
-    return scatter_f64(ds, d_illum, faces, n_cells, k0, k1, d_out, ld_out, (hipStream_t)stream, kernel, tile);
+    return scatter_f64(ds, d_illum, faces, n_cells, n_cells, k0, k1, d_out, ld_out, (hipStream_t)stream, kernel, tile);
 }
 
 static int moments_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, const double* d_normals,
@@ -793,7 +797,7 @@ int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* h
         if (p->upload_pending[i] && i != stream) RSH_HIP_TRY(hipStreamWaitEvent(st, p->upload_ev[i], 0));
     double* dst = p->d_S + (int64_t)(k0 - p->k_begin) * p->n;
     RSH_HIP_TRY(hipEventRecord(p->t0[stream], st));
-    int rc = scatter_f64(p->ds, p->d_illum, p->faces, p->n, k0, k1, dst, p->n, st, RSH_F64_AUTO, nullptr);
+    int rc = scatter_f64(p->ds, p->d_illum, p->faces, p->n, p->n, k0, k1, dst, p->n, st, RSH_F64_AUTO, nullptr);
     if (rc) return rc;
     RSH_HIP_TRY(hipEventRecord(p->t1[stream], st));
     p->timed[stream] = true;
@@ -801,6 +805,61 @@ int rsh_scatter(rsh_problem p, int32_t k0, int32_t k1, int32_t stream, double* h
         RSH_HIP_TRY(hipMemcpyAsync(host_out + (int64_t)(k0 - p->k_begin) * p->n, dst,
                                    (size_t)(k1 - k0) * p->n * sizeof(double), hipMemcpyDeviceToHost, st));
     return RSH_OK;
+}
+
+int rsh_scatter_from_host(rsh_problem p, const double* host_illum, int32_t k0, int32_t k1, double* host_out,
+                          int32_t chunks) {
+    if (!p || !host_illum) return fail(RSH_E_ARG, "null problem or host illum");
+    if (k0 < p->k_begin || k1 > p->k_end || k0 > k1)
+        return fail(RSH_E_ARG, "direction range outside the problem's local range");
+    if (chunks < 1 || chunks > 64) return fail(RSH_E_ARG, "chunks must be 1..64");
+    std::lock_guard<std::mutex> lock(p->mu);
+    RSH_HIP_TRY(hipSetDevice(p->device));
+    // The device illum is overwritten: earlier work on the problem's streams first.
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) RSH_HIP_TRY(hipStreamSynchronize(p->streams[i]));
+    const int64_t n = p->n;
+    const int faces = p->faces, m = p->ds->m;
+    if (n == 0) return RSH_OK;
+    hipStream_t up = p->streams[RSH_STREAM_SCATTERING_1], comp = p->streams[RSH_STREAM_SCATTERING_2],
+                down = p->streams[RSH_STREAM_PARAMS];
+    std::vector<hipEvent_t> ev((size_t)2 * chunks, nullptr);
+    auto cleanup = [&](int status) {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        return status;
+    };
+    for (auto& e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(RSH_E_HIP, "event creation"));
+    const size_t row = (size_t)n * faces * sizeof(double);      // one direction of the illum
+    const size_t srow = (size_t)n * sizeof(double);             // one direction of S
+    double* s_base = p->d_S + (int64_t)(k0 - p->k_begin) * n;
+    double* h_base = host_out ? host_out + (int64_t)(k0 - p->k_begin) * n : nullptr;
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < chunks; c++) {
+        const int64_t c0 = n * c / chunks, c1 = n * (c + 1) / chunks, nc = c1 - c0;
+        if (nc == 0) continue;
+        if ((e = hipMemcpy2DAsync(p->d_illum + c0 * faces, row, host_illum + c0 * faces, row,
+                                  (size_t)nc * faces * sizeof(double), (size_t)m, hipMemcpyHostToDevice, up)) !=
+                hipSuccess ||
+            (e = hipEventRecord(ev[(size_t)(2 * c)], up)) != hipSuccess ||
+            (e = hipStreamWaitEvent(comp, ev[(size_t)(2 * c)], 0)) != hipSuccess)
+            return cleanup(hip_fail(e, "chunk upload"));
+        if (k1 == k0) continue;
+        const int rc = scatter_f64(p->ds, p->d_illum + c0 * faces, faces, nc, n, k0, k1, s_base + c0, n, comp,
+                                   RSH_F64_AUTO, nullptr);
+        if (rc) return cleanup(rc);
+        if (!h_base) continue;
+        if ((e = hipEventRecord(ev[(size_t)(2 * c + 1)], comp)) != hipSuccess ||
+            (e = hipStreamWaitEvent(down, ev[(size_t)(2 * c + 1)], 0)) != hipSuccess ||
+            (e = hipMemcpy2DAsync(h_base + c0, srow, s_base + c0, srow, (size_t)nc * sizeof(double),
+                                  (size_t)(k1 - k0), hipMemcpyDeviceToHost, down)) != hipSuccess)
+            return cleanup(hip_fail(e, "chunk copy back"));
+    }
+    for (int i = 0; i < RSH_STREAM_COUNT; i++)
+        if ((e = hipStreamSynchronize(p->streams[i])) != hipSuccess) return cleanup(hip_fail(e, "synchronize"));
+    for (int i = 0; i < RSH_STREAM_COUNT; i++) p->upload_pending[i] = false;
+    return cleanup(RSH_OK);
 }
 
 int rsh_problem_set_geometry(rsh_problem p, const double* normals, const double* areas, const double* volume) {

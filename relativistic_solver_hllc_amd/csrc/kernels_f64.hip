@@ -180,8 +180,8 @@ hipError_t launch_direct_f64(const double* d_illum, int faces, int64_t n, int m,
 // through registers into one of two LDS buffers while the other is consumed.
 // ---------------------------------------------------------------------------
 template <int RB, int W, int SC, int FACES>
-__global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restrict__ illum, int64_t n, int m,
-                                                          const double* __restrict__ panel, int steps,
+__global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restrict__ illum, int64_t n, int64_t ldx,
+                                                          int m, const double* __restrict__ panel, int steps,
                                                           double* __restrict__ out, int64_t ld, int rows) {
     __shared__ v2d lds[2][F64TileShape<RB, W, SC>::UNITS];
     const int lane = threadIdx.x & 63;
@@ -190,8 +190,8 @@ __global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restri
     const int64_t cell = (int64_t)blockIdx.x * (16 * W) + wave * 16 + (lane & 15);
     const int rb0 = blockIdx.y * RB;
     v4d acc[RB];
-    mfma_f64_accumulate<RB, W, SC, FACES>(illum, n, m, panel + (int64_t)blockIdx.y * steps * RB * 64, steps, cell,
-                                          lds, acc);
+    mfma_f64_accumulate<RB, W, SC, FACES>(illum, n, ldx, m, panel + (int64_t)blockIdx.y * steps * RB * 64, steps,
+                                          cell, lds, acc);
     if (cell >= n) return;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
@@ -204,12 +204,12 @@ __global__ __launch_bounds__(64 * W) void mfma_f64_kernel(const double* __restri
 }
 
 template <int W, int SC, int FACES>
-static hipError_t launch_mfma_rb(int rb, dim3 grid, hipStream_t st, const double* illum, int64_t n, int m,
+static hipError_t launch_mfma_rb(int rb, dim3 grid, hipStream_t st, const double* illum, int64_t n, int64_t ldx, int m,
                                  const double* panel, int steps, double* out, int64_t ld, int rows) {
-#define RSH_RB_CASE(V)                                                                                      \
-    case V:                                                                                                 \
-        hipLaunchKernelGGL((mfma_f64_kernel<V, W, SC, FACES>), grid, dim3(64 * W), 0, st, illum, n, m, panel, \
-                           steps, out, ld, rows);                                                           \
+#define RSH_RB_CASE(V)                                                                                           \
+    case V:                                                                                                      \
+        hipLaunchKernelGGL((mfma_f64_kernel<V, W, SC, FACES>), grid, dim3(64 * W), 0, st, illum, n, ldx, m, panel, \
+                           steps, out, ld, rows);                                                                \
         break;
     switch (rb) {
         RSH_RB_CASE(1) RSH_RB_CASE(2) RSH_RB_CASE(3) RSH_RB_CASE(4) RSH_RB_CASE(5) RSH_RB_CASE(6)
@@ -227,10 +227,11 @@ bool mfma_f64_tile_supported(int waves, int chunk_steps) {
     return (waves == 4 && (chunk_steps == 2 || chunk_steps == 4)) || (waves == 8 && chunk_steps == 2);
 }
 
-hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_panel,
+hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int64_t ldx, int m, const double* d_panel,
                            const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves, int chunk_steps,
                            hipStream_t st) {
     if (n <= 0 || rows <= 0) return hipSuccess;
+    if (ldx < n) return hipErrorInvalidValue;
     if (!mfma_f64_tile_supported(waves, chunk_steps) || (faces != 1 && faces != 4)) return hipErrorInvalidValue;
     if (pl.steps % chunk_steps != 0) return hipErrorInvalidValue;
     const int64_t bx = (n + 16 * waves - 1) / (16 * waves);
@@ -239,8 +240,8 @@ hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, c
 #define RSH_TILE(W, SC)                                                                                \
     if (waves == W && chunk_steps == SC) {                                                             \
         if (faces == 4)                                                                                \
-            return launch_mfma_rb<W, SC, 4>(rb, grid, st, d_illum, n, m, d_panel, pl.steps, d_out, ld, rows); \
-        return launch_mfma_rb<W, SC, 1>(rb, grid, st, d_illum, n, m, d_panel, pl.steps, d_out, ld, rows);    \
+            return launch_mfma_rb<W, SC, 4>(rb, grid, st, d_illum, n, ldx, m, d_panel, pl.steps, d_out, ld, rows); \
+        return launch_mfma_rb<W, SC, 1>(rb, grid, st, d_illum, n, ldx, m, d_panel, pl.steps, d_out, ld, rows);    \
     }
     RSH_TILE(8, 2)
     RSH_TILE(4, 2)

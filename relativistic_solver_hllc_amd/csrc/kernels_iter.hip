@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 3 : 2) void grouped_iter_f64_kerne
         double* __restrict__ out = P.I[parity ^ 1];
         const int64_t cell = item.tile * (16 * W) + wave * 16 + (lane & 15);
         v4d acc[RB];
-        mfma_f64_accumulate<RB, W, SC, 1>(in, P.n, P.m, P.panel + (int64_t)item.pass * P.steps * RB * 64, P.steps,
+        mfma_f64_accumulate<RB, W, SC, 1>(in, P.n, P.n, P.m, P.panel + (int64_t)item.pass * P.steps * RB * 64, P.steps,
                                           cell, lds, acc);
         double res = 0.0;
         if (cell < P.n)

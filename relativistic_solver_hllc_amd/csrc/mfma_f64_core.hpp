@@ -26,11 +26,12 @@ struct F64TileShape {
 };
 
 // Accumulates acc[rb] = sum_j panel(row block rb, j) * X[j][cell] over all
-// steps of one pass for the wave's 16 cells. panel_pass points at the pass's
-// panel (steps * RB fragments). Ends with a workgroup barrier, so the LDS
-// buffers may be reused by the caller right after it returns.
+// steps of one pass for the wave's 16 cells (cells < n valid; direction rows
+// of the input ldx cells apart). panel_pass points at the pass's panel (steps *
+// RB fragments). Ends with a workgroup barrier, so the LDS buffers may be
+// reused by the caller right after it returns.
 template <int RB, int W, int SC, int FACES>
-__device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ illum, int64_t n, int m,
+__device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ illum, int64_t n, int64_t ldx, int m,
                                                     const double* __restrict__ panel_pass, int steps,
                                                     int64_t cell, v2d (&lds)[2][F64TileShape<RB, W, SC>::UNITS],
                                                     v4d (&acc)[RB]) {
@@ -48,8 +49,8 @@ __device__ __forceinline__ void mfma_f64_accumulate(const double* __restrict__ i
     // Panel of this pass: steps * RB fragments, stage t = units [t*UNITS, (t+1)*UNITS).
     const gcv2d_p psrc = (gcv2d_p)as_global(panel_pass) + tid;
     // Face pointer of (j = jr, cell); one j-step advances 4 directions.
-    const gcdouble_p xp = as_global(illum) + ((int64_t)jr * n + (cvalid ? cell : 0)) * FACES;
-    const int64_t xstep = 4 * n * FACES;
+    const gcdouble_p xp = as_global(illum) + ((int64_t)jr * ldx + (cvalid ? cell : 0)) * FACES;
+    const int64_t xstep = 4 * ldx * FACES;
 
 #pragma unroll
     for (int i = 0; i < RB; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};

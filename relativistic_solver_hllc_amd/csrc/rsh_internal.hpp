@@ -62,7 +62,7 @@ hipError_t launch_direct_f64(const double* d_illum, int faces, int64_t n, int m,
                              int k0, int k1, double* d_out, int64_t ld, hipStream_t st);
 
 bool mfma_f64_tile_supported(int waves, int chunk_steps);
-hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int m, const double* d_panel,
+hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int64_t ldx, int m, const double* d_panel,
                            const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves,
                            int chunk_steps, hipStream_t st);
 

@@ -429,6 +429,12 @@ class Problem:
     def calculate_int_scattering(self, k0: int, k1: int, stream: int = STREAM_SCATTERING_1, host_out=None) -> None:
         N.call("rsh_scatter", self.handle, k0, k1, stream, None if host_out is None else _ptr(host_out))
 
+    def calculate_int_scattering_from_host(self, host_illum, k0: int, k1: int, host_out=None, chunks: int = 8) -> None:
+        """CalculateIntScattering's upload + compute + copy back, pipelined over `chunks` cell ranges
+        (rsh_scatter_from_host); returns when done."""
+        N.call("rsh_scatter_from_host", self.handle, _ptr(host_illum), k0, k1,
+               None if host_out is None else _ptr(host_out), chunks)
+
     def sync_stream(self, stream: int) -> None:
         N.call("rsh_sync", self.handle, stream)
 

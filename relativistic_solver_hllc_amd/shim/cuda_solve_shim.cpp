@@ -242,9 +242,9 @@ int CalculateIntScatteringAsync(const grid_directions_t&, grid_t& grid) {
 // Reference: cuda/cuda_solve.cu:153-170: upload all of grid.Illum, compute, copy back, synchronously.
 int CalculateIntScattering(const grid_directions_t&, grid_t& grid) {
     require_init("CalculateIntScattering");
-    check(rsh_copy_illum_sync(g.problem, grid.Illum, (int64_t)4 * g.n_dirs * g.n_cells), "CalculateIntScattering");
-    scatter_local(grid, 0, g.k_end - g.k_begin, RSH_STREAM_SCATTERING_1);
-    check(rsh_sync(g.problem, RSH_STREAM_SCATTERING_1), "CalculateIntScattering");
+    // upload, integral and copy back overlapped over cell ranges; returns when done
+    check(rsh_scatter_from_host(g.problem, grid.Illum, g.k_begin, g.k_end, grid.scattering, 8),
+          "CalculateIntScattering");
     return 0;
 }
 

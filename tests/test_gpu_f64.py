@@ -203,6 +203,50 @@ def test_problem_sync_copy_and_full_range(gpu):
     assert rel(out, orc.scatter_reference(illum, xyz, w, F)) < 1e-12
 
 
+@pytest.mark.parametrize("chunks", [1, 3, 8])
+def test_scatter_from_host_pipelined(gpu, chunks):
+    """CalculateIntScattering's upload + compute + copy back pipelined by cell range
+    (rsh_scatter_from_host): S for a local direction range equals the unpipelined
+    sequence bit for bit (same kernel, same per-cell sums) and the oracle to 1e-12;
+    the device illum holds the whole upload afterwards (a later scatter of the
+    full range from it matches too). Ragged cell ranges (n not a multiple of chunks)."""
+    n, m = 2999, 60
+    kb, ke = 5, 50
+    rng = np.random.default_rng(11)
+    illum = rng.lognormal(0.0, 1.0, size=(m, n, 4))
+    xyz, w, F = rs.fibonacci_directions(m)
+    host = D.PinnedArray((illum.size,))
+    host.a[:] = illum.reshape(-1)
+    a = D.PinnedArray(((ke - kb) * n,))
+    b = D.PinnedArray(((ke - kb) * n,))
+    a.a[:] = -1.0
+    b.a[:] = -2.0
+    p = rs.Problem(n, xyz, w, F, kb, ke)
+    try:
+        p.calculate_int_scattering_from_host(host, kb + 3, ke, host_out=a, chunks=chunks)
+        p.copy_illum_on_device(host, illum.size)
+        p.calculate_int_scattering(kb + 3, ke, stream=0, host_out=b)
+        p.sync_stream(0)
+        got_a = a.a.reshape(ke - kb, n).copy()
+        got_b = b.a.reshape(ke - kb, n).copy()
+        # rows before kb + 3 are not written by either call
+        assert np.all(got_a[:3] == -1.0) and np.all(got_b[:3] == -2.0)
+        assert np.array_equal(got_a[3:], got_b[3:])
+        # the device illum is complete after the pipelined call: recompute from it
+        p.calculate_int_scattering_from_host(host, kb, kb, host_out=None, chunks=chunks)
+        p.calculate_int_scattering(kb, ke, stream=1, host_out=b)
+        p.sync_stream(1)
+        full = b.a.reshape(ke - kb, n).copy()
+    finally:
+        p.close()
+        host.free()
+        a.free()
+        b.free()
+    ref = orc.scatter_reference(illum, xyz, w, F, k_disp=kb, k_count=ke - kb, threads=16)
+    assert rel(got_a[3:], ref[3:]) < 1e-12
+    assert rel(full, ref) < 1e-12
+
+
 @pytest.mark.big
 def test_1572864x400_full_size(gpu):
     """64^3 x 6 = 1,572,864 cells, M = 400, Thomson, log-normal(0, 1.5) (20.1 GB of faces)."""
