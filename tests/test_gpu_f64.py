@@ -287,3 +287,25 @@ def test_1572864x400_full_size(gpu):
         illum.free()
         out.free()
         out2.free()
+
+
+def test_scatter_from_host_cell_level_state(gpu):
+    """The pipelined host path on a cell-level (faces = 1) problem, HG phase with
+    row normalisation, with more cell ranges than a few cells each (chunks = 64
+    on 100 cells) and a host array that is not pinned: equal to the unpipelined
+    sequence bit for bit and to the oracle to 1e-12."""
+    n, m = 100, 33
+    rng = np.random.default_rng(12)
+    cells = rng.lognormal(0.0, 1.0, size=(m, n))
+    xyz, w, F = rs.fibonacci_directions(m)
+    a = np.full((m, n), -1.0)
+    b = np.full((m, n), -2.0)
+    with rs.Problem(n, xyz, w, F, faces=1, phase="hg", g=0.6, normalize=True) as p:
+        p.calculate_int_scattering_from_host(cells, 0, m, host_out=a, chunks=64)
+        p.copy_illum_on_device(cells, cells.size)
+        p.calculate_int_scattering(0, m, stream=2, host_out=b)
+        p.wait()
+    assert np.array_equal(a, b)
+    P = orc.phase_matrix(xyz, w, F, 1, 0.6, True)
+    ref = orc.scatter_matrix(cells[:, :, None], P)
+    assert rel(a, ref) < 1e-12
