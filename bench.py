@@ -340,7 +340,7 @@ def pcie_section(D, rs, illum, xyz, w, F, n, m):
         up, rest = once()
         ok = bool(np.isfinite(s_host.a[:: max(1, m // 7), :: max(1, n // 1000)]).all())
         # CalculateIntScattering's sequence pipelined by cell range (rsh_scatter_from_host)
-        chunks = 8
+        chunks = 32
         p.calculate_int_scattering_from_host(host, 0, m, host_out=s_host, chunks=chunks)
         t0 = time.perf_counter()
         p.calculate_int_scattering_from_host(host, 0, m, host_out=s_host, chunks=chunks)
@@ -357,7 +357,7 @@ def pcie_section(D, rs, illum, xyz, w, F, n, m):
             "upload_ms": up * 1e3, "upload_gbs": m * n * 32 / up / 1e9, "scatter_and_d2h_ms": rest * 1e3,
             "total_ms": total * 1e3, "pairs_per_s_pcie_inclusive": n * m / total,
             "pipelined": {"chunks": chunks, "total_ms": piped * 1e3, "pairs_per_s_pcie_inclusive": n * m / piped,
-                          "how": "upload, integral and copy-back of 8 cell ranges overlapped on three streams "
+                          "how": "upload, integral and copy-back of 32 cell ranges overlapped on three streams "
                                  "(rsh_scatter_from_host)"},
             "finite_output": ok}
 

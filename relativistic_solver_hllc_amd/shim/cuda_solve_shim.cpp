@@ -243,7 +243,7 @@ int CalculateIntScatteringAsync(const grid_directions_t&, grid_t& grid) {
 int CalculateIntScattering(const grid_directions_t&, grid_t& grid) {
     require_init("CalculateIntScattering");
     // upload, integral and copy back overlapped over cell ranges; returns when done
-    check(rsh_scatter_from_host(g.problem, grid.Illum, g.k_begin, g.k_end, grid.scattering, 8),
+    check(rsh_scatter_from_host(g.problem, grid.Illum, g.k_begin, g.k_end, grid.scattering, 32),
           "CalculateIntScattering");
     return 0;
 }
