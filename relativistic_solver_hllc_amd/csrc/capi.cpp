@@ -824,6 +824,9 @@ int rsh_scatter_from_host(rsh_problem p, const double* host_illum, int32_t k0, i
                 down = p->streams[RSH_STREAM_PARAMS];
     std::vector<hipEvent_t> ev((size_t)2 * chunks, nullptr);
     auto cleanup = [&](int status) {
+        // On an error, let the copies already queued finish before the caller reuses its buffers.
+        if (status != RSH_OK)
+            for (int i = 0; i < RSH_STREAM_COUNT; i++) (void)hipStreamSynchronize(p->streams[i]);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         return status;

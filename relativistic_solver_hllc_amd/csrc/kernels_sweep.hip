@@ -17,8 +17,9 @@
 // the pass distance and the neighbour's index in its pass); older upstream
 // values come from global memory, where they are complete by then: a wave's
 // stores of pass q are done once it has waited for a load issued after them,
-// which it does in pass q + 2, before the barrier that ends pass q + 2, and
-// such a value is read in pass q + 4 or later. The full fence (store
+// which it does in pass q + 1, before the barrier that ends pass q + 1, and
+// such a value is read by pass q + 4 or later, whose loads are issued during
+// pass q + 3, after the barrier that ends pass q + 2. The full fence (store
 // completion) only precedes the residual pass.
 //
 // Face values. The reference keeps three node values per face and only ever
@@ -40,12 +41,16 @@
 // records and pairs in one 32-byte record, and its up to nine s values in a
 // fixed 80-byte slot, all in position order. A level is cut into passes of one
 // cell per thread; a pass's position data is fetched a pass ahead, and the
-// by-cell reads (S, upstream entries) of the next pass of the same level are
-// issued before the current pass's stores. Threads past a pass's end repeat its
-// first cell (identical values), so no branch surrounds a load or a store and
-// the wait counters stay exact.
+// by-cell global reads (S, upstream entries older than the ring) of the next
+// pass are issued before the current pass's stores and consumed after its
+// barrier, together with that pass's ring reads. Threads past a pass's end
+// repeat its first cell (identical values), so no branch surrounds a load and
+// the wait counters stay exact. A face out on both sides whose neighbour was
+// swept in the pass just before is the one write that the LDS-only barrier
+// would not order: the host flags such passes, and the pass before completes
+// its stores ahead of the barrier.
 //
-// Measured (196,608 cells x 400 directions): 6.4 ms per sweep (6.9 ms with
+// Measured (196,608 cells x 400 directions): 6.3 ms per sweep (6.9 ms with
 // global memory and a store-completing barrier between levels), of which
 // ~1.2 ms the residual pass; the FP64 arithmetic class barely matters (pure
 // transport, class 3, is 4 % faster than class 1), L2 misses 61 % of requests
@@ -94,8 +99,8 @@ __device__ inline double cur_illum(const double* __restrict__ x, double s, doubl
 // ahead, while the current pass computes).
 struct PassIn {
     SweepPos p;
-    double s[kSweepRecStride];
-    int cnt;  // cells in the pass | last-of-level flag << 31
+    double s[9];
+    int cnt;  // cells in the pass | (complete the previous pass's stores first) << 31
 };
 
 // The pass table is read through the scalar cache (constant address space,
@@ -124,13 +129,17 @@ __device__ inline void load_pass(const SweepDirDev& d, int pi, PassIn& x) {
     for (int f = 0; f < 4; f++) x.p.pr[f] = pp->pr[f];
     x.p.ring[0] = pp->ring[0];
     x.p.ring[1] = pp->ring[1];
+    // the nine s values only: a register loaded with the slot's unused tenth
+    // value and then reused for a temporary would wait for the newest load
+    static_assert(kSweepRecStride == 10, "nine s values in a 10-double slot");
     const auto* sp = gptr(reinterpret_cast<const v2d*>(d.s9 + (int64_t)i * kSweepRecStride));
 #pragma unroll
-    for (int q = 0; q < kSweepRecStride / 2; q++) {
+    for (int q = 0; q < 4; q++) {
         const v2d t = sp[q];
         x.s[2 * q] = t.x;
         x.s[2 * q + 1] = t.y;
     }
+    x.s[8] = gptr(d.s9)[(int64_t)i * kSweepRecStride + 8];
 }
 
 // ReCalcIllum's per-entry term with the reference's comparison (a NaN ratio never wins).
@@ -139,11 +148,12 @@ __device__ inline void residual_term(double old, double cur, double& loc) {
     if (b > loc) loc = b;
 }
 
-// Values a pass reads by cell: S, the upstream entries of its in faces (final
-// once their level has passed the barrier) and, for the residual, the previous
-// intensities of its own entries. Issued for the next pass of the same level
+// Values a pass reads by cell from global memory: S and, for its in faces
+// whose upstream cell was swept at least kSweepRingDelta + 1 passes earlier,
+// that cell's entry (final by then, see the header). Issued for the next pass
 // before the current pass's stores (a load issued after a store could only be
-// waited for together with it), after the barrier at a level change.
+// waited for together with it) and consumed after the barrier, so their
+// latency hides behind the stores and the barrier.
 struct PassDyn {
     double S;
     double fin[4];
@@ -152,29 +162,45 @@ struct PassDyn {
 // LDS ring of the last kSweepRing passes' face values: [slot][face][index].
 typedef double SweepRing[kSweepRing][4][kSweepBlock];
 
-// rp: index of the reading pass (ring slot of an upstream pass = its index mod
-// kSweepRing).
-__device__ inline void load_dyn(const PassIn& x, const double* __restrict__ Sk, const double* fv, PassDyn& y,
-                                const SweepRing& ring, int rp) {
-    typedef double v2d __attribute__((ext_vector_type(2)));
+// Ring code of face f: (pass distance << 9) | upstream index in its pass, or 0
+// when the face is not read from the ring.
+__device__ inline uint32_t ring_code(const SweepPos& p, int f) {
+    return (p.ring[f >> 1] >> (16 * (f & 1))) & 0xffffu;
+}
+
+__device__ inline void load_upstream(const PassIn& x, const double* __restrict__ Sk, const double* fv, PassDyn& y) {
     // Unconditional loads at always-valid addresses (no branch around a load,
     // so the wait counters stay exact): a face whose value comes from the ring,
-    // or that needs none, reads the direction's first entry in global memory,
-    // the same address in every lane, so that such lanes add no line requests
-    // (the value is ignored); the ring read of a face without a ring code reads
-    // slot 0, face 0, index 0 and is ignored too.
+    // or that needs none, reads the direction's first entry, the same address
+    // in every lane, so that such lanes add no line requests (the value is
+    // ignored).
     y.S = Sk[x.p.cell];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const int p = x.p.pr[f];
         const bool up = ((x.p.meta >> f) & 1) && p >= 0;
-        const uint32_t code = (x.p.ring[f >> 1] >> (16 * (f & 1))) & 0xffffu;
-        const bool in_ring = up && code != 0;
-        const double g = fv[up && !in_ring ? (int64_t)p : 0];  // upstream cell's own entry
-        const int slot = in_ring ? (rp - (int)(code >> 9)) & (kSweepRing - 1) : 0;
-        const double l = ring[slot][in_ring ? (p & 3) : 0][in_ring ? (int)(code & 511u) : 0];
-        y.fin[f] = in_ring ? l : g;
+        y.fin[f] = fv[up && ring_code(x.p, f) == 0 ? (int64_t)p : 0];  // upstream cell's own entry
     }
+}
+
+// Upstream values of the in faces of pass rp: from the ring for faces with a
+// ring code (slot of an upstream pass = its index mod kSweepRing), else the
+// global values fetched by load_upstream. Read after the barrier that ends
+// pass rp - 1, before pass rp writes its own slot. All four ring reads are
+// issued unconditionally, before any of them is used (a face without a ring
+// code reads slot 0, face 0, index 0 and ignores it), so their latencies
+// overlap.
+__device__ inline void upstream_values(const SweepPos& p, const PassDyn& y, const SweepRing& ring, int rp,
+                                       double (&v)[4]) {
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const uint32_t code = ring_code(p, f);
+        const bool in_ring = code != 0;
+        const int slot = in_ring ? (rp - (int)(code >> 9)) & (kSweepRing - 1) : 0;
+        v[f] = ring[slot][in_ring ? (p.pr[f] & 3) : 0][in_ring ? (int)(code & 511u) : 0];
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++) v[f] = ring_code(p, f) != 0 ? v[f] : y.fin[f];
 }
 
 // Barrier that orders only LDS accesses: global stores may stay in flight
@@ -209,19 +235,19 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
         // Threads past the pass's cell count repeat its first cell: same inputs,
         // same values written, same residual terms (max is idempotent), so no
         // branch is needed around the loads and stores.
-        const bool last = cur.cnt < 0;  // last pass of its level (uniform)
         const int64_t cell = cur.p.cell;
         const uint32_t meta = cur.p.meta;
         load_pass(d, pi + 1 < d.passes ? pi + 1 : pi, nxt);  // the last pass re-reads itself (unused)
         double own[4];
         {
             double ix[4];
+            upstream_values(cur.p, dc, ring, pi, ix);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const int p = cur.p.pr[f];
                 // in faces: BoundaryConditions value (its 3-node mean is the value itself for 0 and 1)
                 // or the upstream cell's entry
-                ix[f] = p < 0 ? boundary_value(p) : dc.fin[f];
+                ix[f] = p < 0 ? boundary_value(p) : ix[f];
                 own[f] = ix[f];
             }
             uint32_t left = ~meta & 15u;
@@ -245,30 +271,36 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
                 own[3] = f == 3 ? v : own[3];
             }
         }
-        if (!last) load_dyn(nxt, Sk, fv, dn, ring, pi + 1);  // same level: its upstream values are final
+        load_upstream(nxt, Sk, fv, dn);
         {
             v2d* dst = reinterpret_cast<v2d*>(fv + cell * 4);
             dst[0] = v2d{own[0], own[1]};
             dst[1] = v2d{own[2], own[3]};
 #pragma unroll
             for (int f = 0; f < 4; f++) {
-                if ((meta >> (4 + f)) & 1) fv[cur.p.pr[f]] = own[f];  // out on both sides, this cell later
+                // out on both sides, this cell later: the neighbour's entry too
+                // (rare; skipped by the whole wave otherwise - an unconditional
+                // store into the cell's own entry measured 17 % slower)
+                if ((meta >> (4 + f)) & 1) fv[cur.p.pr[f]] = own[f];
             }
             // this pass's values into its ring slot (threads past the pass's end
             // write their own, never-read indices)
 #pragma unroll
             for (int f = 0; f < 4; f++) ring[pi & (kSweepRing - 1)][f][threadIdx.x] = own[f];
         }
+        // the next pass rewrites an entry stored in this one (a face out on
+        // both sides, flagged by the host): complete this pass's stores first,
+        // so that the later write lands last (uniform branch, rare)
+        if (nxt.cnt < 0) __builtin_amdgcn_s_waitcnt(0);
         // every pass ends at a barrier: no wave runs more than a pass ahead, so
         // a ring slot is rewritten only after every read of its previous pass
         lds_barrier();
-        if (last) load_dyn(nxt, Sk, fv, dn, ring, pi + 1);  // next level: after the barrier
     };
     PassIn pa, pb;
     PassDyn da, db;
     if (d.passes > 0) {
         load_pass(d, 0, pa);
-        load_dyn(pa, Sk, fv, da, ring, 0);  // level 0: no upstream values
+        load_upstream(pa, Sk, fv, da);  // level 0: no upstream values
     }
     for (int pi = 0; pi < d.passes; pi += 2) {
         step(pa, da, pb, db, pi);

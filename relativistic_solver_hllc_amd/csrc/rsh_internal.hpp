@@ -112,7 +112,7 @@ struct SweepPos {
 struct SweepDirDev {
     const SweepPos* pos;   // [n] by position in level order
     const double* s9;      // [n][kSweepRecStride] s of record slot q (0 where unused)
-    const int32_t* pass;   // [passes][2]: first position, count | (last pass of its level) << 31
+    const int32_t* pass;   // [passes][2]: first position, count | (complete earlier stores first) << 31
     int32_t passes;
     int32_t levels;
 };

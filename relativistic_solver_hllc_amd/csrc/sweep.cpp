@@ -176,18 +176,19 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
     std::vector<int32_t> pass;
     for (int32_t l = 0; l < levels; l++)
         for (int32_t b0 = level_ptr[(size_t)l]; b0 < level_ptr[(size_t)l + 1]; b0 += kSweepBlock) {
-            const int32_t cnt = std::min(kSweepBlock, level_ptr[(size_t)l + 1] - b0);
-            const bool last = b0 + kSweepBlock >= level_ptr[(size_t)l + 1];
             pass.push_back(b0);
-            pass.push_back(cnt | (last ? (int32_t)0x80000000u : 0));
+            pass.push_back(std::min(kSweepBlock, level_ptr[(size_t)l + 1] - b0));
         }
     const int32_t passes = (int32_t)(pass.size() / 2);
     // Ring codes: an in face whose upstream cell was swept 1..kSweepRingDelta
-    // passes earlier reads that cell's values from the LDS ring.
+    // passes earlier reads that cell's values from the LDS ring. A pass that
+    // rewrites the entry of a cell swept one pass earlier (a face out on both
+    // sides) is flagged: the stores of the pass before it are completed ahead
+    // of its barrier, so that the later write lands last.
     {
         std::vector<int32_t> pass_of((size_t)n), where((size_t)n);
         for (int32_t pi = 0; pi < passes; pi++) {
-            const int32_t b0 = pass[(size_t)(2 * pi)], cnt = pass[(size_t)(2 * pi + 1)] & 0x7fffffff;
+            const int32_t b0 = pass[(size_t)(2 * pi)], cnt = pass[(size_t)(2 * pi + 1)];
             for (int32_t j = 0; j < cnt; j++) pass_of[(size_t)(b0 + j)] = pi;
         }
         for (int64_t i = 0; i < n; i++) where[(size_t)cells[(size_t)i]] = (int32_t)i;
@@ -195,6 +196,9 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
             SweepPos& P = posv[(size_t)i];
             for (int f = 0; f < 4; f++) {
                 const int32_t p = P.pr[f];
+                if (p >= 0 && ((P.meta >> (4 + f)) & 1) &&
+                    pass_of[(size_t)i] - pass_of[(size_t)where[(size_t)(p / 4)]] == 1)
+                    pass[(size_t)(2 * pass_of[(size_t)i] + 1)] |= (int32_t)0x80000000u;
                 if (p < 0 || !((P.meta >> f) & 1)) continue;
                 const int32_t up = where[(size_t)(p / 4)];
                 const int32_t delta = pass_of[(size_t)i] - pass_of[(size_t)up];
