@@ -528,7 +528,8 @@ int rsh_sweep_levels(rsh_sweep s, int32_t k, int32_t* levels);
 /* One sweep over all directions on a stream (NULL = null stream), device
  * arrays: scattering [M][ld] (grid.scattering), new face intensities
  * d_illum_new [M][N][4]; with d_illum_old (distinct, [M][N][4]) also
- * d_norm [M] = ReCalcIllum's max |old - new| / new per direction.
+ * d_norm [M] = ReCalcIllum's max |old - new| / new per direction. Both
+ * illum arrays are 16-byte aligned (RSH_E_ARG otherwise; device allocations are).
  * cls = solve_mode.class_vtk: 0, 1, 2, 3 or 10 (GetCurIllum,
  * Illum_async.cpp:259-430; 11 couples to the gas-dynamics state and is not
  * offered). Replaces the direction loop of MPI_CalculateIllumAsync

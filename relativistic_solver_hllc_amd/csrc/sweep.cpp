@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -249,6 +250,8 @@ int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t 
     if (cls == 0 && !s->d_nodes) return fail(RSH_E_ARG, "class 0 needs the face nodes (rsh_sweep_create nodes)");
     if (d_illum_old && (d_illum_old == d_illum_new || !d_norm))
         return fail(RSH_E_ARG, "the residual needs distinct old and new arrays and a norm array");
+    if ((reinterpret_cast<uintptr_t>(d_illum_new) | reinterpret_cast<uintptr_t>(d_illum_old)) & 15)
+        return fail(RSH_E_ARG, "illum arrays must be 16-byte aligned");
     std::lock_guard<std::mutex> lock(s->mu);
     for (const auto& d : s->dirs)
         if (!d.set) return fail(RSH_E_STATE, "every direction needs rsh_sweep_set_direction first");

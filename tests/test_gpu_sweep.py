@@ -112,6 +112,12 @@ def test_sweep_rejects_bad_input(gpu):
             sp.run(0, d_S, d_new)                                            # class 0 without nodes
         with pytest.raises(rs.N.RshError):
             sp.run(11, d_S, d_new)
+        # illum not 16-byte aligned (rejected on the host; the array has room
+        # for the shifted extent all the same)
+        d_pad = D.DeviceArray((4 * n * 4 + 2,))
+        with pytest.raises(rs.N.RshError):
+            sp.run(1, d_S, d_pad.ptr + 8)
+        d_pad.free()
         d_S.fill(1.0)
         sp.run(1, d_S, d_new)
         D.synchronize()
