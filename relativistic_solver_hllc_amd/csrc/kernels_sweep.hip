@@ -67,7 +67,7 @@ __device__ inline double boundary_value(int type_bound) { return type_bound == -
 
 // GetCurIllum of one class, in the reference's expression order.
 template <int CLS>
-__device__ inline double cur_illum(const double* __restrict__ x, double s, double I_0, double S) {
+__device__ inline double cur_illum(bool in_source, double s, double I_0, double S) {
     double Q, alpha, betta;
     if constexpr (CLS == 2) {
         const double ss = s * 388189 * 1e5;
@@ -77,9 +77,9 @@ __device__ inline double cur_illum(const double* __restrict__ x, double s, doubl
         return I < 0 ? 0 : I;
     } else {
         if constexpr (CLS == 0) {
+            // |x - (0.5, 0, 0)| < 0.05 at the node, evaluated on the host
             Q = 0; alpha = 2; betta = 1;
-            const double dx = x[0] - 0.5, dy = x[1] - 0.0, dz = x[2] - 0.0;
-            if (sqrt(dx * dx + dy * dy + dz * dz) < 0.05) { Q = 0; alpha = 0.5; betta = 0.5; }
+            if (in_source) { Q = 0; alpha = 0.5; betta = 0.5; }
         } else if constexpr (CLS == 1) {
             Q = 1; alpha = 1; betta = alpha / 2;
         } else if constexpr (CLS == 3) {
@@ -195,7 +195,7 @@ __device__ inline void upstream_values(const SweepPos& p, const PassDyn& y, cons
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const uint32_t code = ring_code(p, f);
-        const bool in_ring = code != 0;
+        const bool in_ring = code != 0 && ((p.meta >> f) & 1);  // out faces: class 0 flags
         const int slot = in_ring ? (rp - (int)(code >> 9)) & (kSweepRing - 1) : 0;
         v[f] = ring[slot][in_ring ? (p.pr[f] & 3) : 0][in_ring ? (int)(code & 511u) : 0];
     }
@@ -215,7 +215,6 @@ __device__ inline void lds_barrier() {
 
 template <int CLS>
 __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* __restrict__ dirs, int64_t n,
-                                                            const double* __restrict__ nodes,
                                                             const double* __restrict__ S, int64_t ld,
                                                             const double* __restrict__ old_illum, double* new_illum,
                                                             double* __restrict__ norm) {
@@ -261,8 +260,8 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
                 for (int j = 0; j < 3; j++) {
                     const int g = (meta >> (8 + 2 * (3 * r + j))) & 3;
                     const double ix0 = g == 0 ? ix[0] : g == 1 ? ix[1] : g == 2 ? ix[2] : ix[3];
-                    const double* x = CLS == 0 ? nodes + ((cell * 4 + f) * 3 + j) * 3 : nullptr;
-                    I[j] = cur_illum<CLS>(x, cur.s[3 * r + j], ix0, dc.S);
+                    const bool in_source = (ring_code(cur.p, f) >> j) & 1;  // out face: class 0 flags
+                    I[j] = cur_illum<CLS>(in_source, cur.s[3 * r + j], ix0, dc.S);
                 }
                 const double v = (I[0] + I[1] + I[2]) / 3;
                 own[0] = f == 0 ? v : own[0];
@@ -334,7 +333,7 @@ __global__ __launch_bounds__(kSweepBlock) void sweep_kernel(const SweepDirDev* _
 
 }  // namespace
 
-hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, const double* d_nodes,
+hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n,
                         const double* d_S, int64_t ld, const double* d_old, double* d_new, double* d_norm,
                         hipStream_t st) {
     if (m <= 0 || n <= 0) return hipSuccess;
@@ -345,7 +344,7 @@ hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, co
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(sweep_kernel<C>),                      \
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kRingBytes);             \
         if (e != hipSuccess) return e;                                                                           \
-        hipLaunchKernelGGL(sweep_kernel<C>, grid, block, kRingBytes, st, d_dirs, n, d_nodes, d_S, ld, d_old,     \
+        hipLaunchKernelGGL(sweep_kernel<C>, grid, block, kRingBytes, st, d_dirs, n, d_S, ld, d_old,              \
                            d_new, d_norm);                                                                       \
     } while (0)
     switch (cls) {

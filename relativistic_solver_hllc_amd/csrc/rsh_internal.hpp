@@ -106,7 +106,9 @@ struct SweepPos {
     int32_t pr[4];  // pairs of the cell
     // ring code of in face f in bits 16 (f % 2) of ring[f / 2]: 0 = read the
     // upstream entry from global memory, else (passes back << 9) | index of the
-    // upstream cell in its pass
+    // upstream cell in its pass; for an out face f the same 16 bits hold, in
+    // bit j, class 0's source test for node j of that face (|x - (0.5, 0, 0)|
+    // < 0.05, evaluated once when the direction is set)
     uint32_t ring[2];
 };
 struct SweepDirDev {
@@ -116,7 +118,7 @@ struct SweepDirDev {
     int32_t passes;
     int32_t levels;
 };
-hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n, const double* d_nodes,
+hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n,
                         const double* d_S, int64_t ld, const double* d_old, double* d_new, double* d_norm,
                         hipStream_t st);
 // Resident-panel kernel (cell-level state only): plan and launch. Its result

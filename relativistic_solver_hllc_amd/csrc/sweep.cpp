@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -22,7 +23,9 @@ struct rsh_sweep_s {
     int device = 0;
     int64_t n = 0;
     int m = 0;
-    double* d_nodes = nullptr;
+    // class 0's source test per cell, bit 3 f + j for node j of face f (|x -
+    // (0.5, 0, 0)| < 0.05, GetCurIllum's expression); empty without nodes
+    std::vector<uint16_t> node_in_source;
     std::vector<int32_t> pairs;  // host copy: validation, levels, per-position copies
     struct Dir {
         void* block = nullptr;  // one device allocation per direction
@@ -70,10 +73,13 @@ int rsh_sweep_create(int32_t device, int64_t n_cells, int32_t n_dirs, const int3
     if (hipMalloc(&s->d_table, (size_t)n_dirs * sizeof(SweepDirDev)) != hipSuccess)
         return cleanup(fail(RSH_E_NOMEM, "sweep: device allocation failed"));
     if (nodes) {
-        const size_t b = (size_t)n_cells * 36 * sizeof(double);
-        if (hipMalloc(&s->d_nodes, b) != hipSuccess) return cleanup(fail(RSH_E_NOMEM, "sweep: node allocation failed"));
-        if (hipMemcpy(s->d_nodes, nodes, b, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(RSH_E_HIP, "sweep: node upload failed"));
+        s->node_in_source.assign((size_t)n_cells, 0);
+        for (int64_t c = 0; c < n_cells; c++)
+            for (int b = 0; b < 12; b++) {
+                const double* x = nodes + (c * 12 + b) * 3;
+                const double dx = x[0] - 0.5, dy = x[1] - 0.0, dz = x[2] - 0.0;
+                if (std::sqrt(dx * dx + dy * dy + dz * dz) < 0.05) s->node_in_source[(size_t)c] |= (uint16_t)(1u << b);
+            }
     }
     *out = s;
     return RSH_OK;
@@ -84,7 +90,6 @@ int rsh_sweep_destroy(rsh_sweep s) {
     (void)hipSetDevice(s->device);
     for (auto& d : s->dirs)
         if (d.block) (void)hipFree(d.block);
-    if (s->d_nodes) (void)hipFree(s->d_nodes);
     if (s->d_table) (void)hipFree(s->d_table);
     delete s;
     return RSH_OK;
@@ -166,6 +171,8 @@ int rsh_sweep_set_direction(rsh_sweep s, int32_t k, const int32_t* order, const 
                 const int32_t u = p / 4;
                 if (!((face_state[u] >> (p % 4)) & 1) && pos[(size_t)u] < pos[(size_t)c]) meta |= 1u << (4 + f);
             }
+            if (!s->node_in_source.empty())
+                P.ring[f / 2] |= (uint32_t)((s->node_in_source[(size_t)c] >> (3 * f)) & 7u) << (16 * (f % 2));
             for (int j = 0; j < 3; j++, q++) {
                 const int64_t r = x0_off[(size_t)c] + q;  // records of the cell in out-face order
                 s9[(size_t)(i * kSweepRecStride + q)] = x0_s[r];
@@ -247,7 +254,7 @@ int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t 
     if (!s || !d_scattering || !d_illum_new || ld < s->n) return fail(RSH_E_ARG, "null pointer or ld < n_cells");
     if (cls != 0 && cls != 1 && cls != 2 && cls != 3 && cls != 10)
         return fail(RSH_E_UNSUPPORTED, "class_vtk must be 0, 1, 2, 3 or 10 (11 couples to the gas dynamics state)");
-    if (cls == 0 && !s->d_nodes) return fail(RSH_E_ARG, "class 0 needs the face nodes (rsh_sweep_create nodes)");
+    if (cls == 0 && s->node_in_source.empty()) return fail(RSH_E_ARG, "class 0 needs the face nodes (rsh_sweep_create nodes)");
     if (d_illum_old && (d_illum_old == d_illum_new || !d_norm))
         return fail(RSH_E_ARG, "the residual needs distinct old and new arrays and a norm array");
     if ((reinterpret_cast<uintptr_t>(d_illum_new) | reinterpret_cast<uintptr_t>(d_illum_old)) & 15)
@@ -263,7 +270,7 @@ int rsh_sweep_run(rsh_sweep s, int32_t cls, const double* d_scattering, int64_t 
         RSH_HIP_TRY(hipMemcpy(s->d_table, table.data(), table.size() * sizeof(SweepDirDev), hipMemcpyHostToDevice));
         s->table_dirty = false;
     }
-    RSH_HIP_TRY(launch_sweep(cls, s->d_table, s->m, s->n, s->d_nodes, d_scattering, ld, d_illum_old, d_illum_new,
+    RSH_HIP_TRY(launch_sweep(cls, s->d_table, s->m, s->n, d_scattering, ld, d_illum_old, d_illum_new,
                              d_norm, st));
     return RSH_OK;
 }
