@@ -254,7 +254,10 @@ int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
  * (one per row-block class) captured as a hipGraph, the state stays resident,
  * and problems join and leave between runs (continuous batching).
  *   rsh_batch_footprint  device bytes a problem occupies (admission by HBM;
- *                        slots hold whole row blocks of 16 directions)
+ *                        slots hold whole row blocks of 16 directions; each
+ *                        buffer counted as rsh_batch_reserve's arena carves it,
+ *                        rounded up to 256 bytes); RSH_E_UNSUPPORTED for what a
+ *                        batch cannot hold (FP64 form with n_cells >= 2^24)
  *   rsh_batch_add        copies I0 and B (device, [M][n_cells] FP64) into a slot
  *                        (FP64 form: n_cells < 2^24, else RSH_E_UNSUPPORTED),
  *                        ordered after all work submitted to the null stream
@@ -315,7 +318,8 @@ typedef enum {
     RSH_REQ_RUNNING = 1,
     RSH_REQ_CONVERGED = 2,
     RSH_REQ_MAXED = 3,
-    RSH_REQ_CANCELLED = 4
+    RSH_REQ_CANCELLED = 4,
+    RSH_REQ_FAILED = 5  /* could not be admitted even with no other request resident */
 } rsh_request_status;
 
 typedef struct {

@@ -47,6 +47,7 @@ struct Slot {
     F64Plan plan{};
     SplitPlan splan{};
     bool res = false;  // split form on the resident-panel grouped kernel
+    PanelRef panel;    // the phase panel the device problem table points at
     int iterations = 0;
     int64_t bytes = 0;
 };
@@ -308,10 +309,16 @@ int64_t padded_rows(int form, int m) {
     return form == RSH_FORM_FP64 ? (int64_t)plan_f64(m, m, kGroupedF64MaxRb).rb_total * 16 : (m + 15) / 16 * 16;
 }
 
+int64_t arena_round(int64_t bytes) { return (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+// Device bytes of a slot's buffers as the arena carves them (each buffer
+// rounded up to the arena alignment), so that a problem whose footprint fits the
+// budget fits an empty arena.
 int64_t footprint(int form, int64_t n, int m) {
-    if (form == RSH_FORM_FP64) return 3 * padded_rows(form, m) * n * 8;  // I ping-pong + B, padded rows
+    const int64_t rows = arena_round(padded_rows(form, m) * n * 8);  // B (and FP64 I), padded rows
+    if (form == RSH_FORM_FP64) return 3 * rows;                     // I ping-pong + B
     if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)  // state ping-pong + B + scales
-        return 2 * split_state_units(form, 1, n, m) * 16 + padded_rows(form, m) * n * 8 + n * 4;
+        return 2 * arena_round(split_state_units(form, 1, n, m) * 16) + rows + arena_round(n * 4);
     return -1;
 }
 
@@ -323,6 +330,8 @@ int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes
     if (!bytes || n_cells < 0 || m <= 0) return fail(RSH_E_ARG, "invalid footprint arguments");
     const int64_t f = footprint(form, n_cells, m);
     if (f < 0) return fail(RSH_E_UNSUPPORTED, "form not supported by batches");
+    if (form == RSH_FORM_FP64 && n_cells >= kGroupedF64MaxCells)
+        return fail(RSH_E_UNSUPPORTED, "FP64 batch problems hold fewer than 2^24 cells");
     *bytes = f;
     return RSH_OK;
 }
@@ -490,7 +499,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         free_slot(b, s);
         return hip_fail(e, "batch slot initialisation");
     }
-    const PanelEntry* pe = nullptr;
+    PanelRef pe;
     int rc = get_panel(ds, 0, ds->m, 1, form, b->stream, &pe, s.res ? &s.splan : nullptr, kGroupedF64MaxRb);
     if (rc) {
         free_slot(b, s);
@@ -501,6 +510,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     // batch stream before the next iteration.
     if (!seed) RSH_HIP_TRY(hipStreamSynchronize(b->stream));
     s.active = true;
+    s.panel = pe;
     BatchProblem& P = b->h_probs[id];
     P = BatchProblem{};
     P.B = s.B;

@@ -55,32 +55,10 @@ int check_device(int dev) {
     return RSH_OK;
 }
 
-void free_panel(PanelEntry& pe) {
-    hipFree(pe.d_panel);
-    hipFree(pe.d_row_inv);
-    if (pe.ready) hipEventDestroy(pe.ready);
-}
-
-// Panel for rows [k0,k1), face count and form, built on stream st if absent.
-int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const PanelEntry** out,
-              const SplitPlan* resident, int f64_max_rb) {
-    std::lock_guard<std::mutex> lock(ds->mu);
-    auto key = std::make_tuple(k0, k1, faces, form, form == RSH_FORM_FP64 ? f64_max_rb : (resident ? 1 : 0));
-    auto it = ds->panels.find(key);
-    if (it != ds->panels.end()) {
-        // Make st wait for the build only while it may still be running.
-        if (hipEventQuery(it->second.ready) == hipErrorNotReady)
-            RSH_HIP_TRY(hipStreamWaitEvent(st, it->second.ready, 0));
-        *out = &it->second;
-        return RSH_OK;
-    }
-    if (ds->panels.size() >= 64) {
-        // Bounded cache: drop everything after the device is idle.
-        RSH_HIP_TRY(hipDeviceSynchronize());
-        for (auto& kv : ds->panels) free_panel(kv.second);
-        ds->panels.clear();
-    }
-    PanelEntry pe;
+// Build an entry of the panel cache: on stream st, for rows [k0, k1).
+int build_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, const SplitPlan* resident,
+                int f64_max_rb, PanelEntry& pe) {
+    pe.device = ds->device;
     hipError_t e = hipSuccess;
     if (form == RSH_FORM_FP64) {
         pe.plan = plan_f64(ds->m, k1 - k0, f64_max_rb);
@@ -98,16 +76,59 @@ int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(pe.ready, st);
-    if (e != hipSuccess) {
-        free_panel(pe);
-        return hip_fail(e, "phase panel build");
+    if (e != hipSuccess) return hip_fail(e, "phase panel build");
+    return RSH_OK;
+}
+
+// Panel for rows [k0,k1), face count and form, built on stream st if absent.
+int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, PanelRef* out,
+              const SplitPlan* resident, int f64_max_rb) {
+    std::lock_guard<std::mutex> lock(ds->mu);
+    auto key = std::make_tuple(k0, k1, faces, form, form == RSH_FORM_FP64 ? f64_max_rb : (resident ? 1 : 0));
+    auto it = ds->panels.find(key);
+    if (it != ds->panels.end()) {
+        // Make st wait for the build only while it may still be running.
+        if (hipEventQuery(it->second->ready) == hipErrorNotReady)
+            RSH_HIP_TRY(hipStreamWaitEvent(st, it->second->ready, 0));
+        *out = it->second;
+        return RSH_OK;
     }
-    auto res = ds->panels.emplace(key, pe);
-    *out = &res.first->second;
+    if (ds->panels.size() >= kPanelCacheCap) {
+        // Bounded cache: evict the entries nobody else holds (batch slots and
+        // calls still enqueuing keep theirs); each is freed after the device
+        // is idle, by its destructor.
+        for (auto e = ds->panels.begin(); e != ds->panels.end();) {
+            if (e->second.use_count() == 1)
+                e = ds->panels.erase(e);
+            else
+                ++e;
+        }
+    }
+    auto pe = std::make_shared<PanelEntry>();
+    int rc = build_panel(ds, k0, k1, faces, form, st, resident, f64_max_rb, *pe);
+    if (rc) return rc;
+    ds->panels.emplace(key, pe);
+    *out = pe;
     return RSH_OK;
 }
 
 }  // namespace rsh
+
+PanelEntry::~PanelEntry() {
+    if (d_panel || d_row_inv) {
+        // Work that read the panel may still be queued on any stream of the
+        // device: free only once the device is idle.
+        int cur = -1;
+        const bool have = hipGetDevice(&cur) == hipSuccess;
+        if (hipSetDevice(device) == hipSuccess) {
+            hipDeviceSynchronize();
+            hipFree(d_panel);
+            hipFree(d_row_inv);
+        }
+        if (have && cur != device) hipSetDevice(cur);
+    }
+    if (ready) hipEventDestroy(ready);
+}
 
 namespace rsh {
 
@@ -154,7 +175,7 @@ int scatter_f64(rsh_dirset ds, const double* d_illum, int faces, int64_t n, int6
                                                   : f64_tile_for(ds->device, plan_f64(ds->m, k1 - k0).rb_per_pass);
     if (!mfma_f64_tile_supported(t.waves, t.chunk_steps))
         return fail(RSH_E_UNSUPPORTED, "unsupported MFMA tile (waves, chunk_steps)");
-    const PanelEntry* pe = nullptr;
+    PanelRef pe;  // held until the launch is enqueued
     int rc = get_panel(ds, k0, k1, faces, RSH_FORM_FP64, st, &pe);
     if (rc) return rc;
     RSH_HIP_TRY(launch_mfma_f64(d_illum, faces, n, ldx, ds->m, static_cast<const double*>(pe->d_panel), pe->plan,
@@ -211,7 +232,7 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     const int per_xcd = device_cus(ds->device) / 8;
     if (waves == 0)
         waves = (resident_ok && 4 * sets * rplan.passes >= 3 * per_xcd) ? 1 : split_default_waves();
-    const PanelEntry* pe = nullptr;
+    PanelRef pe;  // held until the launch is enqueued
     if (waves == 1) {
         int rc = get_panel(ds, k0, k1, faces, form, st, &pe, &rplan);
         if (rc) return rc;
@@ -403,7 +424,7 @@ int rsh_dirset_destroy(rsh_dirset ds) {
     if (!ds) return RSH_OK;
     hipSetDevice(ds->device);
     hipDeviceSynchronize();
-    for (auto& kv : ds->panels) free_panel(kv.second);
+    ds->panels.clear();  // entries still held by batch slots live on until those slots go
     hipFree(ds->d_dirs);
     hipFree(ds->d_w);
     hipFree(ds->d_P);

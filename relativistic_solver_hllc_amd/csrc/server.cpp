@@ -173,6 +173,9 @@ int rsh_server_submit(rsh_server srv, const rsh_request_desc* d, int32_t* reques
     if (d->n_cells <= 0 || d->n_dirs <= 0 || !(d->tolerance > 0) || d->max_iterations <= 0)
         return fail(RSH_E_ARG, "invalid request");
     if (d->phase != RSH_PHASE_THOMSON && d->phase != RSH_PHASE_HG) return fail(RSH_E_ARG, "unknown phase function");
+    // Requests the batch cannot hold (unknown form, FP64 with 2^24 cells or
+    // more) or whose footprint (the arena's own rounding included) exceeds the
+    // whole budget are refused here, so that none can wait for ever.
     int64_t bytes = 0;
     int rc = rsh_batch_footprint(d->form, d->n_cells, d->n_dirs, &bytes);
     if (rc) return rc;
@@ -217,19 +220,30 @@ int rsh_server_step(rsh_server srv, int32_t* running, int32_t* finished_now) {
     if (!srv) return fail(RSH_E_ARG, "null server");
     RSH_HIP_TRY(hipSetDevice(srv->device));
     // 1. admission (FIFO, by footprint)
+    auto count_running = [&]() {
+        int k = 0;
+        for (int s : srv->slot_req)
+            if (s >= 0) k++;
+        return k;
+    };
     while (!srv->pending.empty()) {
         const int id = srv->pending.front();
-        const Request& r = srv->reqs[id];
+        Request& r = srv->reqs[id];
         const bool slot_free = std::find(srv->slot_req.begin(), srv->slot_req.end(), -1) != srv->slot_req.end();
         if (!slot_free || srv->used + r.bytes > srv->budget) break;
         int rc = admit(srv, id);
-        if (rc == RSH_E_NOMEM) break;  // arena fragmented: retry after the next retirements
-        if (rc) return rc;
+        if (rc == RSH_E_NOMEM && count_running() > 0) break;  // arena fragmented: retry after retirements
+        if (rc) {
+            // Cannot be admitted even with nothing else resident (or the batch
+            // refused it): the request fails instead of blocking the queue.
+            r.status = RSH_REQ_FAILED;
+            r.t_done = now_s();
+            srv->pending.pop_front();
+            continue;
+        }
         srv->pending.pop_front();
     }
-    int nrun = 0;
-    for (int s : srv->slot_req)
-        if (s >= 0) nrun++;
+    const int nrun = count_running();
     if (running) *running = nrun;
     if (finished_now) *finished_now = 0;
     if (nrun == 0) return RSH_OK;

@@ -319,7 +319,7 @@ class Batch:
             pass
 
 
-REQ_STATUS = {0: "pending", 1: "running", 2: "converged", 3: "maxed", 4: "cancelled"}
+REQ_STATUS = {0: "pending", 1: "running", 2: "converged", 3: "maxed", 4: "cancelled", 5: "failed"}
 
 
 def request(n_cells: int, m: int, phase: str = "thomson", g: float = 0.0, albedo: float = 0.5,

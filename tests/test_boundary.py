@@ -163,6 +163,37 @@ def test_split_footprint_pads_source_rows(form):
     assert rs.Batch.footprint(form, n, 17) - rs.Batch.footprint(form, n, 16) == 16 * n * 8
 
 
+@pytest.mark.parametrize("form", ["fp64", "bf16x2", "fp16x2", "bf16x3"])
+def test_footprint_is_what_the_arena_carves(form):
+    """The footprint counts every slot buffer rounded up to the arena's 256-byte
+    alignment, as rsh_batch_reserve's arena carves it: with an odd cell count a
+    request whose footprint equals the budget still fits an empty arena."""
+    def r256(b):
+        return -(-b // 256) * 256
+
+    n, m = 1001, 200
+    if form == "fp64":
+        want = 3 * r256(208 * n * 8)                  # I ping-pong + B, 13-row-block passes
+    else:
+        nt = 3 if form == "bf16x3" else 2
+        steps = (m + 31) // 32
+        state = -(-n // 16) * steps * nt * 64 * 16    # packed state units of 16 bytes
+        want = 2 * r256(state) + r256(208 * n * 8) + r256(4 * n)
+    assert rs.Batch.footprint(form, n, m) == want
+
+
+def test_footprint_refuses_what_a_batch_cannot_hold():
+    """FP64 slots address one row block with 32-bit offsets: 2^24 cells or more
+    are refused by the footprint itself (so a server rejects such a request at
+    submission instead of queueing it for ever); split forms accept them."""
+    from relativistic_solver_hllc_amd._native import RshError
+
+    with pytest.raises(RshError):
+        rs.Batch.footprint("fp64", 1 << 24, 200)
+    assert rs.Batch.footprint("fp64", (1 << 24) - 1, 16) > 0
+    assert rs.Batch.footprint("bf16x3", 1 << 24, 200) > 0
+
+
 def test_import_fails_loudly_without_the_library(tmp_path):
     """A copy of the package without librsh_scatter.so refuses to load (no Python
     or CPU fallback): importing its scatter module raises ImportError naming the

@@ -321,3 +321,39 @@ def test_fp64_batch_cell_limit(gpu):
         b.run(2)
         r, it = b.residuals()
         assert it[s] == 2 and np.isfinite(r[s])
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16x3"])
+def test_batch_keeps_its_panel_through_cache_eviction(gpu, form):
+    """A batch slot holds the phase panel its device problem table points at.
+    Filling the direction set's panel cache past its capacity (one entry per
+    single-direction output range of scatter_f64) evicts only entries nobody
+    else holds; device memory allocated and overwritten afterwards (where a freed
+    panel could land) must not change the batch's iterations: the state equals
+    that of a batch of the same problem run without the evictions."""
+    m, n = 200, 3000
+    p = make(m, n, "hg", 0.6, seed=41)
+    q = make(m, n, "hg", 0.6, seed=41)
+    with rs.Batch(2) as b, rs.Batch(2) as ref:
+        dI, dB = D.DeviceArray.from_host(p["I0"]), D.DeviceArray.from_host(p["B"])
+        s = b.add(p["ds"], n, dI, dB, 0.5, form=form)
+        r = ref.add(q["ds"], n, dI, dB, 0.5, form=form)
+        b.run(2)
+        b.sync()
+        illum, out = D.DeviceArray((m, n)), D.DeviceArray((m, n))
+        illum.fill(1.0)
+        for k in range(80):                      # 80 distinct panels > the cache's 64
+            p["ds"].scatter_f64(illum, 1, n, k, k + 1, out)
+        D.synchronize()
+        junk = [D.DeviceArray((m * m,)) for _ in range(96)]
+        for a in junk:
+            a.fill(float("nan"))
+        D.synchronize()
+        b.run(4)
+        ref.run(6)
+        got, want = state(b, s, p), state(ref, r, q)
+        for a in junk + [illum, out, dI, dB]:
+            a.free()
+    assert np.array_equal(got, want)
+    p["ds"].close()
+    q["ds"].close()

@@ -112,3 +112,32 @@ def test_poisson_trace(gpu, form, tol):
             info = srv.poll(i)
             assert info["status"] == "converged" and info["residual"] < tol
     assert np.all(lat > 0) and np.all(np.isfinite(lat)) and wall > 0
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16x3"])
+def test_request_filling_the_whole_budget_is_admitted(gpu, form):
+    """The footprint counts the arena's 256-byte rounding of every slot buffer:
+    a request with an odd cell count whose footprint equals the budget is
+    admitted into the empty arena and converges (it used to wait for ever)."""
+    n, m = 2001, 200
+    fp = rs.Batch.footprint(form, n, m)
+    desc = rs.request(n, m, "hg", 0.6, tolerance=1e-6, form=form, seed=12)
+    with rs.Server(fp, 4, iters_per_step=2) as srv:
+        rid = srv.submit(desc)
+        drain(srv, limit=100)
+        assert srv.poll(rid)["status"] == "converged"
+
+
+def test_unservable_requests_are_refused_at_submission(gpu):
+    """A request the batch cannot hold (FP64 with 2^24 cells or more) or larger
+    than the whole budget is refused by submit, so nothing can block the queue."""
+    from relativistic_solver_hllc_amd._native import RshError
+
+    with rs.Server(1 * GB, 4, iters_per_step=2) as srv:
+        with pytest.raises(RshError) as e:
+            srv.submit(rs.request(1 << 24, 200, tolerance=1e-8))
+        assert e.value.status == 6
+        with pytest.raises(RshError) as e:
+            srv.submit(rs.request(1 << 20, 800, tolerance=1e-8))
+        assert e.value.status == 3
+        assert srv.stats()["pending"] == 0
