@@ -42,56 +42,118 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(illum_dev, xyz, w, F, n, m, threads, k_slice, variant):
-    """Times the oracle's reference-path restatement on k_slice output directions."""
+def host_cpu_info() -> dict:
+    """CPU model, logical CPUs (nproc), CPUs this process may run on, physical cores."""
+    model, phys = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = core = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    core = v
+                elif not k and pid is not None:
+                    phys.add((pid, core))
+                    pid = core = None
+            if pid is not None:
+                phys.add((pid, core))
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "physical_cores": len(phys) or None}
+
+
+def cpu_baseline(illum_dev, xyz, w, F, n, m, threads, k_slice, cell_frac):
+    """Times the oracle's restatement of the reference CPU path (GetS + CalculateIntCPU,
+    solve_module/illum/Illum_async.cpp:201-256: OpenMP over output directions, then cells, then
+    input directions) on a slice: k_slice output directions x the first n / cell_frac cells of the
+    full mesh (same array strides) x all m input directions; 1 warm-up and 5 timed runs, median,
+    for the -O3 and -Ofast (the reference's host flag) builds on `threads` threads and the -O3
+    build on one thread (one output direction). The cost is exactly linear in output directions
+    and cells (every (cell, direction) pair is the same 4 m loads and m phase evaluations), so
+    each median is scaled to one full evaluation by (m / directions) x (n / cells)."""
     from oracle import orc
 
     log(f"[bench] cpu baseline: copying {illum_dev.nbytes / 1e9:.1f} GB of faces to host")
     host = illum_dev.to_host()
+    c1 = n // cell_frac
     S = np.zeros((k_slice, n))
-    orc.load(variant)
-    orc.scatter_reference(host[:, :1024], xyz, w, F, k_disp=0, k_count=1, threads=1, variant=variant)  # warm
-    t0 = time.perf_counter()
-    orc.scatter_reference(host, xyz, w, F, k_disp=0, k_count=k_slice, threads=threads, variant=variant, out=S)
-    dt = time.perf_counter() - t0
+
+    def median_s(variant, nthreads, kcount):
+        orc.load(variant)
+        times = []
+        for r in range(6):  # 1 warm-up + 5 timed
+            t0 = time.perf_counter()
+            orc.scatter_reference(host, xyz, w, F, k_disp=0, k_count=kcount, cell_begin=0, cell_end=c1,
+                                  threads=nthreads, variant=variant, out=S)
+            if r:
+                times.append(time.perf_counter() - t0)
+        return float(np.median(times)), times
+
+    res = {}
+    for label, variant, nthreads, kcount in (("o3", "o3", threads, k_slice), ("ofast", "ofast", threads, k_slice),
+                                             ("o3_1core", "o3", 1, 1)):
+        med, times = median_s(variant, nthreads, kcount)
+        full_s = med * (m / kcount) * (n / c1)
+        res[label] = {"pairs_per_s": n * m / full_s, "threads": nthreads, "output_directions": kcount,
+                      "cells": c1, "median_s": med, "runs_s": times, "full_evaluation_s": full_s}
+        log(f"[bench] cpu baseline {label}: {nthreads} threads, {kcount} dirs x {c1} cells: median {med:.3f} s "
+            f"-> {n * m / full_s:.3e} pairs/s")
     del host
-    full_s = dt * (m / k_slice)
+    info = host_cpu_info()
     return {
-        "value": n * m / full_s,
+        "value": res["o3"]["pairs_per_s"],
         "unit": "pairs/s",
         "cores": threads,
         "kind": "port",
-        "build": {"parity": "g++ -O2", "o3": "g++ -O3 -march=x86-64-v3", "ofast": "g++ -Ofast -march=x86-64-v3"}[
-            variant],
-        "sample": f"{k_slice} of {m} output directions x all {n} cells x {m} input directions "
-                  f"(one OpenMP thread per output direction, the reference's schedule); "
-                  f"{dt:.2f} s measured, scaled x{m / k_slice:g} to one full evaluation ({full_s:.1f} s)",
+        "build": "g++ -O3 -march=x86-64-v3 -fopenmp (value); -Ofast and 1-core figures beside it",
+        "host": info,
+        "ofast": res["ofast"],
+        "o3": res["o3"],
+        "one_core": res["o3_1core"],
+        "sample": f"{k_slice} of {m} output directions x cells [0, {c1}) of {n} (full-mesh strides) x {m} input "
+                  f"directions on {threads} threads (one OpenMP thread per output direction, the reference's "
+                  f"schedule); 1 warm-up + 5 timed runs, median, scaled x{m / k_slice:g} x{n / c1:g} to one full "
+                  f"evaluation; one-core figure: 1 output direction, same cells",
     }
 
 
 def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cells):
     """Henyey-Greenstein workload: same mesh and intensities, HG g = 0.8 with FP64 row normalisation;
-    FP64 MFMA vs the split forms (state packed once, untimed; FP32 MFMA accumulation)."""
+    FP64 MFMA vs the split forms (FP32 MFMA accumulation). The FP64 result is pinned to the oracle on
+    sampled cells; every split result (all n x m of them) is compared with it on the device."""
     from oracle import orc
 
     res = {"workload": "same 1,572,864-cell x 4-face mesh and intensities, M=400, "
                        "Henyey-Greenstein g=0.8, FP64 row normalisation",
            "phase": "hg", "g": 0.8, "forms": {}}
     ds = rs.DirectionSet(xyz, w, F, phase="hg", g=0.8, normalize=True)
+    ref64 = D.DeviceArray((m, n))
     e0, e1 = D.Event(), D.Event()
     for _ in range(warmup):
-        ds.scatter_f64(illum, 4, n, 0, m, out, stream=stream)
+        ds.scatter_f64(illum, 4, n, 0, m, ref64, stream=stream)
     e0.record(stream)
     for _ in range(steps):
-        ds.scatter_f64(illum, 4, n, 0, m, out, stream=stream)
+        ds.scatter_f64(illum, 4, n, 0, m, ref64, stream=stream)
     e1.record(stream)
     t64 = e0.elapsed_ms(e1) / steps
     res["fp64_ms"] = t64
-    wide = np.arange(0, n, 997)
-    ref64 = out.gather_columns(wide)
     P = orc.phase_matrix(xyz, w, F, 1, 0.8, True)
     ref_or = orc.scatter_matrix(illum.gather_cells(cells), P, threads=16)
-    log(f"[bench] HG g=0.8 FP64: {t64:.3f} ms")
+    r64 = ref64.gather_columns(cells)
+    res["fp64_vs_oracle"] = {"sampled_cells": int(len(cells)),
+                             "max_rel_err": float(np.max(np.abs(r64 - ref_or) / ref_or)), "tolerance": 1e-12}
+    log(f"[bench] HG g=0.8 FP64: {t64:.3f} ms, sampled max rel err vs oracle "
+        f"{res['fp64_vs_oracle']['max_rel_err']:.2e}")
     out32 = D.DeviceArray((m, n), np.float32)
 
     def timed(fn):
@@ -106,12 +168,12 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
         return t_ms
 
     def errors(o):
+        full = rs.rel_error_dev(o, ref64, m, n, threshold=1e-6, stream=stream)
         got_or = o.gather_columns(cells).astype(np.float64)
         r_or = np.abs(got_or - ref_or) / ref_or
-        got = o.gather_columns(wide).astype(np.float64)
-        r = np.abs(got - ref64) / ref64
-        return {"max_rel_err_vs_oracle": float(r_or.max()), "mean_rel_err_vs_oracle": float(r_or.mean()),
-                "max_rel_err_vs_fp64_gpu": float(r.max()), "mean_rel_err_vs_fp64_gpu": float(r.mean())}
+        return {"max_rel_err": full["max_rel_err"], "mean_rel_err": full["mean_rel_err"],
+                "results_above_1e-6": full["count_above"], "results_compared": full["results"],
+                "max_rel_err_vs_oracle_sampled": float(r_or.max())}
 
     # Resident split state: the face average packed once (as the source
     # iteration keeps it), contracted by the default split kernel.
@@ -119,11 +181,12 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
         sb, cb = rs.split_sizes(form, 1, n, m)
         state = D.DeviceArray((sb,), np.uint8)
         scale = D.DeviceArray((cb // 4,), np.float32)
-        e0.record(stream)
-        rs.pack_split_avg_dev(form, illum, n, m, state, scale, stream=stream)
-        e1.record(stream)
-        pack_ms = e0.elapsed_ms(e1)
+        rs.pack_split_avg_dev(form, illum, n, m, state, scale, stream=stream)  # warm-up
+        pack_ms = timed(lambda: rs.pack_split_avg_dev(form, illum, n, m, state, scale, stream=stream))
+        pack_bytes = m * n * 4 * 8 + sb + n * 4
         entry = {"state": "cell-level (face average packed once)", "state_bytes": sb, "pack_ms": pack_ms,
+                 "pack_hbm_gbs_algorithmic": pack_bytes / (pack_ms * 1e-3) / 1e9,
+                 "pack": "one pass: faces read once, FP64 average, cell scale, MFMA-layout parts",
                  "kernel": "resident-panel"}
         for label, o, ob in (("fp64_out", out, 8), ("fp32_out", out32, 4)):
             t = timed(lambda: ds.scatter_split(form, state, scale, 1, n, 0, m, o, out_f32=(ob == 4), stream=stream))
@@ -131,10 +194,14 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
             entry[label] = {"ms": t, "speedup_vs_fp64": t64 / t, "pairs_per_s": n * m / (t * 1e-3),
                             "hbm_gbs_algorithmic": byts / (t * 1e-3) / 1e9,
                             "roofline_frac_hbm": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": byts,
-                            **errors(o)}
-            log(f"[bench] {form} cell state {label}: {t:.3f} ms ({t64 / t:.2f}x FP64), "
-                f"max rel err vs oracle {entry[label]['max_rel_err_vs_oracle']:.2e}")
-        entry["err_sample"] = f"{len(cells)} cells x {m} dirs vs oracle; {len(wide)} cells x {m} dirs vs FP64 GPU"
+                            "end_to_end_ms_from_fp64_faces": pack_ms + t,
+                            "end_to_end_speedup_vs_fp64": t64 / (pack_ms + t), **errors(o)}
+            log(f"[bench] {form} cell state {label}: {t:.3f} ms ({t64 / t:.2f}x FP64), pack {pack_ms:.3f} ms; "
+                f"all {n * m} results: max rel err {entry[label]['max_rel_err']:.2e}, "
+                f"mean {entry[label]['mean_rel_err']:.2e}, above 1e-6: {entry[label]['results_above_1e-6']}")
+        entry["err"] = (f"every one of the {n * m} results vs the FP64 MFMA result (itself within "
+                        f"{res['fp64_vs_oracle']['max_rel_err']:.1e} of the oracle on {len(cells)} sampled cells); "
+                        f"also vs the oracle on those sampled cells")
         res["forms"][form] = entry
         state.free()
         scale.free()
@@ -153,6 +220,7 @@ def split_section(D, rs, illum, out, xyz, w, F, n, m, steps, warmup, stream, cel
         state.free()
         scale.free()
     out32.free()
+    ref64.free()
     ds.close()
     return res
 
@@ -277,7 +345,7 @@ def sweep_section(D, rs, edge=32, m=400, steps=5, cpu=True):
     byts = m * n * (4 + 8 + 1 + 8) + records * 9 + in_interior * 8 + m * n * 4 * 8 * 3
     cpu_res = None
     if cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(16, host_cpu_info()["affinity_cpus"])
         c0 = time.perf_counter()
         _, _ = orc.sweep(per, pairs, S, old, cls=1, threads=threads, variant="o3")
         c_s = time.perf_counter() - c0
@@ -394,8 +462,18 @@ def batch_section(D, rs, iters=20):
                        "B ~ U(0.5,2), albedo 0.5, 20 iterations",
            "problems": len(probs), "cells_total": sum(p["n"] for p in probs), "pairs_per_iteration": pairs,
            "iterations": iters,
-           "fp64_grouped_tile": "4 waves x 2 j-steps per LDS stage, <= 13 row blocks (208 rows) per pass",
+           "fp64_grouped_tile": "autotuned (below), <= 13 row blocks (208 rows) per pass",
            "forms": {}}
+    # Grouped-launch configurations chosen by the autotuner on this device (untimed):
+    # FP64 per row-block class (M = 200, 400, 800 share one), split forms per M.
+    t0 = time.perf_counter()
+    tuned = {"fp64": {"400": rs.batch_autotune("fp64", 400)[0][1]}}
+    for form in ("bf16x2", "fp16x2", "bf16x3"):
+        tuned[form] = {str(mm): rs.batch_autotune(form, mm)[0][1] for mm in sorted({p["m"] for p in probs})}
+    res["autotune"] = {"choices": tuned, "seconds": time.perf_counter() - t0,
+                       "candidates": "FP64: MFMA tiles (waves, j-steps per LDS stage) (4,2), (4,4), (8,2); "
+                                     "split: resident-panel vs streaming-panel grouped kernel"}
+    log(f"[bench] batch autotune: {tuned}")
     states = {}
     for form in ("fp64", "bf16x2", "fp16x2", "bf16x3"):
         b = rs.Batch(len(probs))
@@ -476,7 +554,8 @@ def main() -> int:
     ap.add_argument("--tile", type=str, default="", help="waves,chunk_steps (default: library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-slice", type=int, default=16)
+    ap.add_argument("--cpu-slice", type=int, default=40, help="output directions of the CPU slice (at least)")
+    ap.add_argument("--cpu-cell-frac", type=int, default=8, help="the CPU slice covers n / this many cells")
     ap.add_argument("--no-moments", action="store_true", help="skip the radiation-moments section")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision (HG g=0.8) section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven problems section")
@@ -571,9 +650,11 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        k_slice = min(args.cpu_slice, m)
-        cpu = cpu_baseline(illum, xyz, w, F, n, m, threads, k_slice, "o3")
+        # This job's CPU share of the box: the CPUs it may run on, at most 16 (one GPU's share).
+        threads = args.cpu_threads or min(16, host_cpu_info()["affinity_cpus"])
+        # at least max(40, threads) output directions, a whole number per thread
+        k_slice = min(m, -(-max(args.cpu_slice, threads) // threads) * threads)
+        cpu = cpu_baseline(illum, xyz, w, F, n, m, threads, k_slice, args.cpu_cell_frac)
         log(f"[bench] cpu baseline: {cpu['value']:.3e} pairs/s on {threads} threads")
 
     sweep = None

@@ -173,7 +173,9 @@ int rsh_moments_f64_dev(rsh_dirset ds, const double* d_illum, int32_t faces, int
  * then the contraction reads only that state:
  *   rsh_split_sizes        bytes of the state and of the per-cell scale array
  *   rsh_pack_split_dev     FP64 illum [M][n_cells][faces] -> state + cell scales
- *   rsh_pack_split_avg_dev FP64 illum [M][n_cells][4] -> cell-level state (faces = 1)
+ *   rsh_pack_split_avg_dev FP64 illum [M][n_cells][4] -> cell-level state (faces = 1);
+ *                          cell-level packs (this one, and rsh_pack_split_dev with
+ *                          faces = 1) read the input once for M <= 1280
  *                          of the face average (summed and halved twice in FP64)
  *   rsh_scatter_split_dev  S rows [k0,k1) from the state, FP32 MFMA accumulation,
  *                          written as FP64 (out_f32 = 0) or FP32 (out_f32 = 1)
@@ -226,6 +228,21 @@ int rsh_dev_scale_f64(double* d, int64_t count, double factor, void* stream);
 /* Plain streaming copy kernel (16-byte loads/stores); reads and writes
  * count*8 bytes, used to calibrate HBM byte counters. */
 int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream);
+/* Relative error of every result against an FP64 reference on the device:
+ * max and mean over rows x n_cells of |got - ref| / |ref| (got FP64, or FP32
+ * when got_f32; row r at d_got + r * ld_got and d_ref + r * ld_ref elements),
+ * and how many exceed `threshold`; a NaN counts as an infinite error.
+ * Synchronous on `stream`. No reference counterpart: it states the split
+ * forms' error per result over a whole output instead of on sampled cells. */
+int rsh_dev_rel_error(const void* d_got, int32_t got_f32, int64_t ld_got, const double* d_ref, int64_t ld_ref,
+                      int64_t rows, int64_t n_cells, double threshold, double* max_rel, double* mean_rel,
+                      int64_t* count_above, void* stream);
+/* Achievable matrix-core rate of the device, TFLOP/s, from register-only loops
+ * of independent MFMAs (8 waves per CU, no memory traffic): kind 0 =
+ * v_mfma_f64_16x16x4f64 (the FP64 kernels' instruction), 1 =
+ * v_mfma_f32_16x16x32_bf16 (the BF16 split forms'). Measures the peak that
+ * the roofline fractions divide by; no reference counterpart. */
+int rsh_mfma_peak(int32_t device, int32_t kind, double* tflops);
 int rsh_dev_synchronize(int32_t device);
 int rsh_dev_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
 int rsh_stream_create(int32_t device, void** out);
@@ -281,6 +298,31 @@ int rsh_batch_destroy(rsh_batch b);
  * 2 = streaming panel. Both give bit-identical states and residuals. No
  * reference counterpart (batching is an extension). */
 int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
+
+/* Autotuner of the grouped iteration launches. No reference counterpart (the
+ * reference's launch shape is fixed: 32 x 16 threads, cuda/cuda_solve.cu:98-99).
+ *   rsh_batch_autotune    times every candidate configuration of the batch
+ *                         class of (form, m) on a seeded problem of n_cells
+ *                         cells, `iterations` graph-replayed iterations each,
+ *                         and records the fastest for the device: FP64 classes
+ *                         (row blocks per pass) choose the MFMA tile (waves,
+ *                         j-steps per LDS stage) among (4,2), (4,4), (8,2);
+ *                         split forms choose, per M, the resident-panel
+ *                         ({1,0}) or streaming-panel ({2,0}) grouped kernel.
+ *                         Classes (FP64) and slots (split) created afterwards
+ *                         in any batch on the device use the choice; every
+ *                         candidate gives bit-identical states.
+ *   rsh_batch_tuned       the recorded choice (the default if none: FP64 (4,2);
+ *                         split {0,0} = resident panel where it applies)
+ *   rsh_batch_set_f64_tile forces the FP64 tile of the classes this batch
+ *                         creates from now on (NULL or {0,0}: tuned or default)
+ *   rsh_batch_slot_config what a slot runs with: its class's FP64 tile, or
+ *                         {1,0} / {2,0} for the split kernels */
+int rsh_batch_autotune(int32_t device, int32_t form, int32_t m, int64_t n_cells, int32_t iterations, rsh_tile* best,
+                       float* best_ms);
+int rsh_batch_tuned(int32_t device, int32_t form, int32_t m, rsh_tile* choice);
+int rsh_batch_set_f64_tile(rsh_batch b, const rsh_tile* tile);
+int rsh_batch_slot_config(rsh_batch b, int32_t slot, rsh_tile* config);
 
 /* Reserves one device arena of `bytes` for the batch's slot buffers (the batch
  * must be empty): later adds carve their buffers from it first-fit instead of

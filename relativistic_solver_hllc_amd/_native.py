@@ -22,6 +22,7 @@ lib = C.CDLL(LIB_PATH)
 i32, i64, u32, u64, f64, f32 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double, C.c_float
 vp, cp = C.c_void_p, C.c_char_p
 P_i32, P_f64, P_f32, P_vp = C.POINTER(i32), C.POINTER(f64), C.POINTER(f32), C.POINTER(vp)
+P_i64 = C.POINTER(i64)
 
 
 class Tile(C.Structure):
@@ -99,6 +100,10 @@ PROTOTYPES = {
     "rsh_batch_add_seeded": (C.c_int, [vp, vp, i64, u64, f64, i32, P_i32]),
     "rsh_batch_reserve": (C.c_int, [vp, i64]),
     "rsh_batch_set_split_kernel": (C.c_int, [vp, i32]),
+    "rsh_batch_autotune": (C.c_int, [i32, i32, i32, i64, i32, C.POINTER(Tile), P_f32]),
+    "rsh_batch_tuned": (C.c_int, [i32, i32, i32, C.POINTER(Tile)]),
+    "rsh_batch_set_f64_tile": (C.c_int, [vp, C.POINTER(Tile)]),
+    "rsh_batch_slot_config": (C.c_int, [vp, i32, C.POINTER(Tile)]),
     "rsh_batch_remove": (C.c_int, [vp, i32]),
     "rsh_batch_run": (C.c_int, [vp, i32]),
     "rsh_batch_sync": (C.c_int, [vp]),
@@ -115,6 +120,8 @@ PROTOTYPES = {
     "rsh_dev_fill_f64": (C.c_int, [vp, i64, f64, vp]),
     "rsh_dev_scale_f64": (C.c_int, [vp, i64, f64, vp]),
     "rsh_dev_copy_f64": (C.c_int, [vp, vp, i64, vp]),
+    "rsh_mfma_peak": (C.c_int, [i32, i32, P_f64]),
+    "rsh_dev_rel_error": (C.c_int, [vp, i32, i64, vp, i64, i64, i64, f64, P_f64, P_f64, P_i64, vp]),
     "rsh_dev_synchronize": (C.c_int, [i32]),
     "rsh_dev_mem_info": (C.c_int, [i32, C.POINTER(i64), C.POINTER(i64)]),
     "rsh_stream_create": (C.c_int, [i32, P_vp]),

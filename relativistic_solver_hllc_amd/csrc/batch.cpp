@@ -20,6 +20,8 @@
 #include <iterator>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "rsh_objects.hpp"
@@ -30,6 +32,27 @@ namespace {
 
 constexpr int kMaxClasses = 16;
 constexpr int kXcdLists = 8;  // per-XCD work lists of the resident-panel split classes
+
+// Grouped-launch tuning table, filled by rsh_batch_autotune: FP64 classes by
+// (device, form, row blocks per pass) -> MFMA tile; split forms by (device,
+// form, M) -> kernel (tile.waves: 1 resident panel, 2 streaming panel).
+std::mutex g_gtune_mu;
+std::map<std::tuple<int, int, int>, rsh_tile> g_gtune;
+
+int tune_key(int form, int m) { return form == RSH_FORM_FP64 ? plan_f64(m, m, kGroupedF64MaxRb).rb_per_pass : m; }
+
+bool grouped_tuned(int device, int form, int key, rsh_tile* t) {
+    std::lock_guard<std::mutex> lock(g_gtune_mu);
+    auto it = g_gtune.find({device, form, key});
+    if (it == g_gtune.end()) return false;
+    *t = it->second;
+    return true;
+}
+
+void set_grouped_tuned(int device, int form, int key, rsh_tile t) {
+    std::lock_guard<std::mutex> lock(g_gtune_mu);
+    g_gtune[{device, form, key}] = t;
+}
 
 struct Slot {
     bool active = false;
@@ -80,6 +103,7 @@ struct rsh_batch_s {
     unsigned long long* d_resid = nullptr;  // 2 * capacity
     unsigned* d_counters = nullptr;         // kMaxClasses * kXcdLists
     int split_kernel = 0;                   // 0 auto, 1 resident panel, 2 streaming panel
+    rsh_tile f64_tile{0, 0};                // FP64 grouped tile of new classes; {0, 0} = tuned or default
     std::map<std::pair<int, int>, ClassState> classes;  // by (form, row-block class)
     bool lists_dirty = true;
     bool graph_valid = false;
@@ -194,7 +218,13 @@ int rebuild(rsh_batch b) {
             cs.form = s.form;
             cs.rb = rb;
             if (s.form == RSH_FORM_FP64) {
+                // forced for this batch, else tuned for the class, else the default
                 cs.tile = rsh_tile{kGroupedF64Waves, kGroupedF64ChunkSteps};
+                rsh_tile tuned{};
+                if (b->f64_tile.waves > 0)
+                    cs.tile = b->f64_tile;
+                else if (grouped_tuned(b->device, s.form, rb, &tuned))
+                    cs.tile = tuned;
                 cs.grid = grouped_iter_f64_resident(rb, cs.tile.waves, cs.tile.chunk_steps, b->device);
             } else if (s.res) {
                 cs.res = true;
@@ -428,7 +458,11 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     s.albedo = albedo;
     s.plan = plan_f64(ds->m, ds->m, kGroupedF64MaxRb);
     s.splan = plan_split(ds->m, ds->m);
-    if (form != RSH_FORM_FP64 && b->split_kernel != 2 && n_cells < (int64_t(1) << 24)) {
+    int split_kernel = b->split_kernel;
+    rsh_tile tuned{};
+    if (form != RSH_FORM_FP64 && split_kernel == 0 && grouped_tuned(b->device, form, ds->m, &tuned))
+        split_kernel = tuned.waves == 2 ? 2 : 0;  // tuned streaming, else auto (resident where it applies)
+    if (form != RSH_FORM_FP64 && split_kernel != 2 && n_cells < (int64_t(1) << 24)) {
         // resident-panel grouped kernel where its plan fits and is specialised
         SplitPlan rplan{};
         int sets = 0;
@@ -436,7 +470,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
             iter_split_res_supported(form, rplan.rb_per_pass, rplan.steps)) {
             s.splan = rplan;
             s.res = true;
-        } else if (b->split_kernel == 1) {
+        } else if (split_kernel == 1) {
             return fail(RSH_E_UNSUPPORTED, "resident-panel iteration kernel: no specialised plan for this M");
         }
     }
@@ -491,9 +525,12 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     }
     if (e == hipSuccess && form != RSH_FORM_FP64) {
         // padding positions of the other buffer are never written by the epilogue
-        if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess &&
-            (e = launch_cell_scale(src_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
-            e = launch_pack_state(form, src_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+        if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess) {
+            if (pack_fused_supported(ds->m))  // scale (I0 and B bound every iterate) and pack in one pass
+                e = launch_pack_fused(form, src_I0, s.B, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+            else if ((e = launch_cell_scale(src_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
+                e = launch_pack_state(form, src_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+        }
     }
     if (e != hipSuccess) {
         free_slot(b, s);
@@ -550,6 +587,114 @@ int rsh_batch_add(rsh_batch b, rsh_dirset ds, int64_t n_cells, const double* d_I
 int rsh_batch_add_seeded(rsh_batch b, rsh_dirset ds, int64_t n_cells, uint64_t seed, double albedo, int32_t form,
                          int32_t* slot) {
     return add_impl(b, ds, n_cells, albedo, form, nullptr, nullptr, &seed, slot);
+}
+
+int rsh_batch_set_f64_tile(rsh_batch b, const rsh_tile* tile) {
+    if (!b) return fail(RSH_E_ARG, "null batch");
+    rsh_tile t = tile ? *tile : rsh_tile{0, 0};
+    if (t.waves != 0 && !((t.waves == 8 && t.chunk_steps == 2) || (t.waves == 4 && (t.chunk_steps == 2 ||
+                                                                                         t.chunk_steps == 4))))
+        return fail(RSH_E_UNSUPPORTED, "grouped FP64 tiles: (8,2), (4,2), (4,4); (0,0) = tuned or default");
+    b->f64_tile = t;
+    return RSH_OK;
+}
+
+int rsh_batch_slot_config(rsh_batch b, int32_t slot, rsh_tile* config) {
+    if (!b || !config || slot < 0 || slot >= b->capacity || !b->slots[slot].active)
+        return fail(RSH_E_ARG, "invalid slot");
+    const Slot& s = b->slots[slot];
+    if (s.form != RSH_FORM_FP64) {
+        *config = rsh_tile{s.res ? 1 : 2, 0};
+        return RSH_OK;
+    }
+    RSH_HIP_TRY(hipSetDevice(b->device));
+    if (b->lists_dirty) {
+        int rc = rebuild(b);
+        if (rc) return rc;
+    }
+    auto it = b->classes.find(std::make_pair(s.form, slot_class(s)));
+    if (it == b->classes.end()) return fail(RSH_E_STATE, "slot has no class");
+    *config = it->second.tile;
+    return RSH_OK;
+}
+
+int rsh_batch_tuned(int32_t device, int32_t form, int32_t m, rsh_tile* choice) {
+    if (!choice || m <= 0 || footprint(form, 1, 1) < 0) return fail(RSH_E_ARG, "invalid arguments");
+    if (!grouped_tuned(device, form, tune_key(form, m), choice))
+        *choice = form == RSH_FORM_FP64 ? rsh_tile{kGroupedF64Waves, kGroupedF64ChunkSteps} : rsh_tile{0, 0};
+    return RSH_OK;
+}
+
+int rsh_batch_autotune(int32_t device, int32_t form, int32_t m, int64_t n_cells, int32_t iterations, rsh_tile* best,
+                       float* best_ms) {
+    if (!best || !best_ms || m <= 0 || n_cells <= 0 || iterations <= 0 || footprint(form, 1, 1) < 0)
+        return fail(RSH_E_ARG, "invalid autotune arguments");
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    std::vector<double> xyz(3 * (size_t)m), w(m);
+    double F = 0;
+    rsh_fibonacci_directions(m, xyz.data(), w.data(), &F);
+    rsh_dirset ds = nullptr;
+    if ((rc = rsh_dirset_create(device, m, xyz.data(), w.data(), F, RSH_PHASE_HG, 0.6, 1, &ds))) return rc;
+    std::vector<rsh_tile> cands;
+    if (form == RSH_FORM_FP64)
+        cands = {{4, 2}, {4, 4}, {8, 2}};
+    else
+        cands = {{1, 0}, {2, 0}};  // resident-panel, streaming-panel grouped kernel
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e;
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        rsh_dirset_destroy(ds);
+        return hip_fail(e, "autotune events");
+    }
+    float bms = -1.f;
+    rsh_tile bt = cands[0];
+    for (const rsh_tile& t : cands) {
+        // A fresh batch per candidate (a class keeps the tile it was created with).
+        rsh_batch b = nullptr;
+        if ((rc = rsh_batch_create(device, 1, 1, &b))) break;
+        int slot = -1;
+        if (form == RSH_FORM_FP64)
+            rc = rsh_batch_set_f64_tile(b, &t);
+        else
+            rc = rsh_batch_set_split_kernel(b, t.waves);
+        if (!rc) rc = rsh_batch_add_seeded(b, ds, n_cells, 0x7E57ull, 0.5, form, &slot);
+        if (rc == RSH_E_UNSUPPORTED) {  // candidate does not apply (e.g. no resident plan for this M)
+            rsh_batch_destroy(b);
+            rc = RSH_OK;
+            continue;
+        }
+        float ms = 0.f;
+        if (!rc) rc = rsh_batch_run(b, 2);  // graph capture, first launches
+        if (!rc) rc = rsh_batch_sync(b);
+        if (!rc) {
+            if ((e = hipEventRecord(e0, b->stream)) != hipSuccess) rc = hip_fail(e, "event");
+        }
+        if (!rc) rc = rsh_batch_run(b, iterations);
+        if (!rc) {
+            if ((e = hipEventRecord(e1, b->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+                rc = hip_fail(e, "event timing");
+        }
+        rsh_batch_destroy(b);
+        if (rc) break;
+        ms /= iterations;
+        if (bms < 0.f || ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    rsh_dirset_destroy(ds);
+    if (rc) return rc;
+    if (bms < 0.f) return fail(RSH_E_UNSUPPORTED, "no grouped kernel applies to this form and M");
+    set_grouped_tuned(device, form, tune_key(form, m), bt);
+    *best = bt;
+    *best_ms = bms;
+    return RSH_OK;
 }
 
 int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind) {

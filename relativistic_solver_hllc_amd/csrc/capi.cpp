@@ -490,6 +490,10 @@ int rsh_pack_split_dev(int32_t form, const double* d_illum, int32_t faces, int64
     if (n_cells == 0) return RSH_OK;
     if (!d_illum || !d_state || !d_cell_scale) return fail(RSH_E_ARG, "null device pointer");
     hipStream_t st = (hipStream_t)stream;
+    if (faces == 1 && pack_fused_supported(m)) {  // one pass over the input
+        RSH_HIP_TRY(launch_pack_fused(form, d_illum, nullptr, 1, n_cells, m, d_cell_scale, d_state, st));
+        return RSH_OK;
+    }
     RSH_HIP_TRY(launch_cell_scale(d_illum, nullptr, n_cells, m, faces, d_cell_scale, st));
     RSH_HIP_TRY(launch_pack_state(form, d_illum, faces, n_cells, m, d_cell_scale, d_state, st));
     return RSH_OK;
@@ -501,6 +505,10 @@ int rsh_pack_split_avg_dev(int32_t form, const double* d_illum, int64_t n_cells,
     if (n_cells == 0) return RSH_OK;
     if (!d_illum || !d_state || !d_cell_scale) return fail(RSH_E_ARG, "null device pointer");
     hipStream_t st = (hipStream_t)stream;
+    if (pack_fused_supported(m)) {  // one pass over the input
+        RSH_HIP_TRY(launch_pack_fused(form, d_illum, nullptr, 4, n_cells, m, d_cell_scale, d_state, st));
+        return RSH_OK;
+    }
     RSH_HIP_TRY(launch_cell_scale(d_illum, nullptr, n_cells, m, 4, d_cell_scale, st));
     RSH_HIP_TRY(launch_pack_state_avg(form, d_illum, n_cells, m, d_cell_scale, d_state, st));
     return RSH_OK;
@@ -634,6 +642,72 @@ int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream
     if (count < 0 || (count > 0 && (!dst || !src))) return fail(RSH_E_ARG, "invalid copy arguments");
     if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(RSH_E_ARG, "copy kernel needs 16-byte aligned pointers");
     RSH_HIP_TRY(launch_copy_f64(dst, src, count, (hipStream_t)stream));
+    return RSH_OK;
+}
+
+int rsh_dev_rel_error(const void* d_got, int32_t got_f32, int64_t ld_got, const double* d_ref, int64_t ld_ref,
+                      int64_t rows, int64_t n_cells, double threshold, double* max_rel, double* mean_rel,
+                      int64_t* count_above, void* stream) {
+    if (!max_rel || !mean_rel || !count_above || rows < 0 || n_cells < 0 || ld_got < n_cells || ld_ref < n_cells)
+        return fail(RSH_E_ARG, "invalid relative-error arguments");
+    *max_rel = 0.0;
+    *mean_rel = 0.0;
+    *count_above = 0;
+    if (rows == 0 || n_cells == 0) return RSH_OK;
+    if (!d_got || !d_ref) return fail(RSH_E_ARG, "null device pointer");
+    const int gx = (int)std::min<int64_t>((n_cells + 255) / 256, 64);
+    const int gy = (int)std::min<int64_t>(rows, 128);
+    double* d_part = nullptr;
+    RSH_HIP_TRY(hipMalloc(&d_part, (size_t)gx * gy * 3 * sizeof(double)));
+    hipError_t e = launch_rel_error(d_got, got_f32 ? 1 : 0, ld_got, d_ref, ld_ref, rows, n_cells, threshold, d_part,
+                                    gx, gy, (hipStream_t)stream);
+    std::vector<double> part((size_t)gx * gy * 3);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e == hipSuccess) e = hipMemcpy(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d_part);
+    if (e != hipSuccess) return hip_fail(e, "relative-error reduction");
+    double mx = 0.0, sum = 0.0, above = 0.0;
+    for (size_t b = 0; b < part.size(); b += 3) {
+        mx = std::max(mx, part[b]);
+        sum += part[b + 1];
+        above += part[b + 2];
+    }
+    *max_rel = mx;
+    *mean_rel = sum / ((double)rows * (double)n_cells);
+    *count_above = (int64_t)above;
+    return RSH_OK;
+}
+
+int rsh_mfma_peak(int32_t device, int32_t kind, double* tflops) {
+    if (!tflops || (kind != 0 && kind != 1)) return fail(RSH_E_ARG, "kind: 0 = FP64 16x16x4, 1 = BF16 16x16x32");
+    *tflops = 0.0;
+    int rc = check_device(device);
+    if (rc) return rc;
+    RSH_HIP_TRY(hipSetDevice(device));
+    const int blocks = 2 * device_cus(device);   // 8 waves per CU: 2 per SIMD
+    const int iters = kind == 0 ? 2000 : 8000;   // ~1 ms per launch at the vendor peak
+    const int reps = 5;
+    double* sink = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&sink, (size_t)blocks * 256 * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_mfma_peak(kind, blocks, iters, sink, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_mfma_peak(kind, blocks, iters, sink, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipFree(sink);
+    if (st) hipStreamDestroy(st);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "MFMA peak loop");
+    const double flops = (double)blocks * 4 * iters * mfma_peak_flops_per_wave_iter(kind) * reps;
+    *tflops = flops / (ms * 1e-3) / 1e12;
     return RSH_OK;
 }
 

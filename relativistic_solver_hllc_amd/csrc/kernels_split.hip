@@ -133,6 +133,90 @@ __global__ __launch_bounds__(256) void pack_state_avg_kernel(const double* __res
     for (int t = 0; t < NT; t++) out[t * 64] = __builtin_bit_cast(v4u, buf[t]);
 }
 
+// Fused cell scale + cell-level pack, one pass over the input: workgroup = one
+// 16-cell tile. Every (direction, cell) of the tile is read once (F faces as
+// 16-byte loads: 16 cells x F faces are contiguous per direction); its
+// face average ((f0 + f1) + f2) + f3) * 1/4 in FP64 (F = 4; the value itself
+// for F = 1) goes to LDS, its largest |face| (and |extra| when given, F = 1)
+// into the cell's maximum. Then the cell scale is set exactly as
+// cell_scale_kernel sets it and the tile's state vectors are written from LDS
+// as pack_state_avg_kernel / pack_state_kernel<F = 1> write them: the same bits
+// as those two passes, with the input read once instead of twice.
+template <int FMT, int NT, int F>
+__global__ __launch_bounds__(256) void pack_fused_kernel(const double* __restrict__ illum,
+                                                         const double* __restrict__ extra, int64_t n, int m,
+                                                         int steps, float* __restrict__ inv_scale,
+                                                         v4u* __restrict__ state) {
+    typedef typename Fmt<FMT>::T T;
+    extern __shared__ double avg[];  // [steps * 32][16]: face averages, zero past m
+    __shared__ double cmax[4][16];
+    const int t = threadIdx.x;
+    const int cl = t & 15;
+    const int64_t ct = blockIdx.x;
+    const int64_t cell = ct * 16 + cl;
+    const bool cvalid = cell < n;
+    const int64_t cell_c = cvalid ? cell : n - 1;  // tail threads re-read a valid cell, results unused
+    const int jrows = steps * 32;
+    double mx = 0.0;
+    // Phase 1: thread t reads directions j = t / 16 + 16 i of cell cl.
+    constexpr int UNR = 5;
+    for (int j0 = t >> 4; j0 < jrows; j0 += 16 * UNR) {
+        double a[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int j = j0 + 16 * u;
+            a[u] = 0.0;
+            if (j < m) {
+                const double* p = illum + ((int64_t)j * n + cell_c) * F;
+                if constexpr (F == 4) {
+                    const double2 lo = *reinterpret_cast<const double2*>(p);
+                    const double2 hi = *reinterpret_cast<const double2*>(p + 2);
+                    a[u] = (((lo.x + lo.y) + hi.x) + hi.y) * 0.25;
+                    mx = fmax(mx, fmax(fmax(fabs(lo.x), fabs(lo.y)), fmax(fabs(hi.x), fabs(hi.y))));
+                } else {
+                    a[u] = p[0];
+                    mx = fmax(mx, fabs(p[0]));
+                    if (extra) mx = fmax(mx, fabs(extra[(int64_t)j * n + cell_c]));
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int j = j0 + 16 * u;
+            if (j < jrows) avg[j * 16 + cl] = cvalid ? a[u] : 0.0;
+        }
+    }
+    // Phase 2: the cell's maximum over the 16 threads that read it.
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    if ((t & 63) < 16) cmax[t >> 6][cl] = mx;
+    __syncthreads();
+    const double cm = fmax(fmax(cmax[0][cl], cmax[1][cl]), fmax(cmax[2][cl], cmax[3][cl]));
+    const int e = cm > 0.0 ? (kTargetExp - 1 - ilogb(cm)) : 0;
+    const float inv = ldexpf(1.0f, -e);
+    if (t < 16 && cvalid) inv_scale[cell] = inv;
+    const double scale = cvalid ? 1.0 / (double)inv : 0.0;
+    // Phase 3: state vectors of the tile: (K-step s, lane) slots, lane l holding
+    // directions 32 s + 8 (l / 16) + e of cell l % 16.
+    v4u* out = state + (int64_t)ct * steps * NT * 64;
+    for (int slot = t; slot < steps * 64; slot += 256) {
+        const int s = slot >> 6, lane = slot & 63;
+        const int j0 = 32 * s + 8 * (lane >> 4);
+        // lane % 16 == t % 16: this thread's own cell and scale
+        T buf[NT][8];
+#pragma unroll
+        for (int e8 = 0; e8 < 8; e8++) {
+            const float x = (float)(avg[(j0 + e8) * 16 + (lane & 15)] * scale);
+            T parts[NT];
+            split_parts<FMT, NT>(x, parts);
+#pragma unroll
+            for (int q = 0; q < NT; q++) buf[q][e8] = parts[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) out[(s * NT + q) * 64 + lane] = __builtin_bit_cast(v4u, buf[q]);
+    }
+}
+
 // ------------------------------------------------------------------ phase panel
 // One block per output row: row scale from the row maximum, then the fragments.
 template <int FMT, int NT>
@@ -284,6 +368,33 @@ hipError_t launch_pack_state_avg(int form, const double* d_illum, int64_t n, int
     else
         hipLaunchKernelGGL((pack_state_avg_kernel<0, 3>), grid, block, 0, st, d_illum, n, m, steps, d_inv, out, total);
     return hipGetLastError();
+}
+
+bool pack_fused_supported(int m) { return (size_t)((m + 31) / 32) * 32 * 16 * sizeof(double) <= 160 * 1024; }
+
+hipError_t launch_pack_fused(int form, const double* d_illum, const double* d_extra, int faces, int64_t n, int m,
+                             float* d_inv, void* d_state, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!pack_fused_supported(m) || (faces != 1 && faces != 4) || (faces == 4 && d_extra))
+        return hipErrorInvalidValue;
+    const int steps = (m + 31) / 32;
+    const size_t lds = (size_t)steps * 32 * 16 * sizeof(double);
+    const dim3 grid((unsigned)((n + 15) / 16)), block(256);
+    v4u* out = static_cast<v4u*>(d_state);
+    const int fmt = split_fmt(form), nt = split_terms(form);
+#define RSH_PF(FMT, NT, F)                                                                                       \
+    {                                                                                                            \
+        auto k = pack_fused_kernel<FMT, NT, F>;                                                                  \
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),                                    \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                \
+        if (e != hipSuccess) return e;                                                                           \
+        hipLaunchKernelGGL(k, grid, block, lds, st, d_illum, d_extra, n, m, steps, d_inv, out);                   \
+        return hipGetLastError();                                                                                \
+    }
+    if (fmt == 0 && nt == 2) { if (faces == 4) RSH_PF(0, 2, 4) else RSH_PF(0, 2, 1) }
+    if (fmt == 1 && nt == 2) { if (faces == 4) RSH_PF(1, 2, 4) else RSH_PF(1, 2, 1) }
+    if (faces == 4) RSH_PF(0, 3, 4) else RSH_PF(0, 3, 1)
+#undef RSH_PF
 }
 
 hipError_t launch_split_panel(int form, const double* d_P, int m, int k0, int k1, double factor, const SplitPlan& pl,

@@ -66,6 +66,17 @@ hipError_t launch_mfma_f64(const double* d_illum, int faces, int64_t n, int64_t 
                            const F64Plan& pl, int rows, double* d_out, int64_t ld, int waves,
                            int chunk_steps, hipStream_t st);
 
+// Relative-error reduction of results against an FP64 reference (gx x gy
+// workgroups, 3 doubles of partial result each: max, sum, count above).
+hipError_t launch_rel_error(const void* got, int got_f32, int64_t ld_got, const double* ref, int64_t ld_ref,
+                            int64_t rows, int64_t n, double threshold, double* partial, int gx, int gy,
+                            hipStream_t st);
+
+// Register-only MFMA throughput loops (kernels_peak.hip): kind 0 FP64
+// 16x16x4, kind 1 BF16 16x16x32; blocks of 4 waves, `iters` iterations each.
+hipError_t launch_mfma_peak(int kind, int blocks, int iters, double* sink, hipStream_t st);
+double mfma_peak_flops_per_wave_iter(int kind);
+
 // ------------------------------------------------------------------ split forms
 struct SplitPlan {
     int rb_per_pass;
@@ -129,6 +140,14 @@ bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int*
 hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const float* d_cell_inv,
                                  const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int sets, int rows,
                                  void* d_out, int64_t ld, int out_f32, hipStream_t st);
+// Cell scale and cell-level state in one pass over the input (faces = 4: the
+// face average; faces = 1: the values, with d_extra joining the scale's
+// maximum); bit-identical to launch_cell_scale followed by
+// launch_pack_state_avg (faces = 4) or launch_pack_state (faces = 1). Needs the
+// tile's averages in LDS: M <= 1280.
+bool pack_fused_supported(int m);
+hipError_t launch_pack_fused(int form, const double* d_illum, const double* d_extra, int faces, int64_t n, int m,
+                             float* d_inv, void* d_state, hipStream_t st);
 // Packs the face average of 4-face FP64 input as a cell-level state.
 hipError_t launch_pack_state_avg(int form, const double* d_illum, int64_t n, int m, const float* d_inv,
                                  void* d_state, hipStream_t st);

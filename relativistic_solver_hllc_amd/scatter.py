@@ -248,6 +248,27 @@ def tuned_tile_f64(m: int, device: int = 0) -> Tuple[int, int]:
     return t.waves, t.chunk_steps
 
 
+def batch_autotune(form: str, m: int, n_cells: int = 100000, iterations: int = 6, device: int = 0):
+    """Times the grouped-launch candidates of the batch class of (form, m) and records the fastest
+    for the device: FP64 -> ("tile", (waves, j-steps)); split forms -> ("kernel", "resident" | "streaming").
+    Returns (choice, ms per iteration)."""
+    t, ms = N.Tile(), C.c_float()
+    N.call("rsh_batch_autotune", device, FORMS[form], m, int(n_cells), iterations, C.byref(t), C.byref(ms))
+    return _grouped_choice(form, t), ms.value
+
+
+def batch_tuned(form: str, m: int, device: int = 0):
+    t = N.Tile()
+    N.call("rsh_batch_tuned", device, FORMS[form], m, C.byref(t))
+    return _grouped_choice(form, t)
+
+
+def _grouped_choice(form: str, t):
+    if form == "fp64":
+        return ("tile", (t.waves, t.chunk_steps))
+    return ("kernel", {0: "auto", 1: "resident", 2: "streaming"}[t.waves])
+
+
 class Batch:
     """Uneven source-iteration problems sharing grouped launches (rsh_batch_*)."""
 
@@ -278,6 +299,17 @@ class Batch:
     def set_split_kernel(self, kind: str) -> None:
         """Kernel for split-form slots added afterwards: "auto", "resident" or "streaming"."""
         N.call("rsh_batch_set_split_kernel", self.handle, {"auto": 0, "resident": 1, "streaming": 2}[kind])
+
+    def set_f64_tile(self, tile: Optional[Tuple[int, int]]) -> None:
+        """FP64 grouped tile (waves, j-steps) of the classes created from now on; None = tuned or default."""
+        t = N.Tile(*(tile or (0, 0)))
+        N.call("rsh_batch_set_f64_tile", self.handle, C.byref(t))
+
+    def slot_config(self, slot: int, form: str):
+        """What a slot runs with: its FP64 class's tile, or the split kernel."""
+        t = N.Tile()
+        N.call("rsh_batch_slot_config", self.handle, slot, C.byref(t))
+        return _grouped_choice(form, t)
 
     def reserve(self, nbytes: int) -> None:
         """One device arena for the slot buffers (batch must be empty)."""
@@ -398,6 +430,25 @@ class Server:
 def fill_uniform_dev(out, lo: float, hi: float, seed: int, stream=None) -> None:
     N.call("rsh_fill_uniform_dev", _ptr(out), out.numel(), float(lo), float(hi), int(seed) & (2**64 - 1),
            _stream(stream))
+
+
+def mfma_peak(kind: str = "fp64", device: int = 0) -> float:
+    """Measured register-only MFMA rate in TFLOP/s ("fp64": 16x16x4 FP64, "bf16": 16x16x32 BF16)."""
+    t = C.c_double()
+    N.call("rsh_mfma_peak", device, {"fp64": 0, "bf16": 1}[kind], C.byref(t))
+    return t.value
+
+
+def rel_error_dev(got, ref, rows: int, n_cells: int, ld_got: Optional[int] = None, ld_ref: Optional[int] = None,
+                  threshold: float = 1e-6, stream=None) -> dict:
+    """Relative error |got - ref| / |ref| of every one of rows x n_cells results (got FP64 or FP32,
+    ref FP64, both device arrays) reduced on the device: max, mean and the count above threshold."""
+    got_f32 = 1 if np.dtype(got.dtype) == np.float32 else 0
+    mx, mean, above = C.c_double(), C.c_double(), C.c_int64()
+    N.call("rsh_dev_rel_error", _ptr(got), got_f32, int(ld_got or n_cells), _ptr(ref), int(ld_ref or n_cells),
+           int(rows), int(n_cells), float(threshold), C.byref(mx), C.byref(mean), C.byref(above), _stream(stream))
+    return {"max_rel_err": mx.value, "mean_rel_err": mean.value, "count_above": above.value,
+            "threshold": threshold, "results": int(rows) * int(n_cells)}
 
 
 # ------------------------------------------------------------------ problem (InitDevice ...)

@@ -129,7 +129,7 @@ def test_autotune_records_a_supported_tile(gpu):
 
 # ---------------------------------------------------------------- split-precision state
 # State error bound after the iterations below, vs the FP64 oracle iteration (relative):
-SPLIT_TOL = {"fp16x2": 2e-6, "bf16x3": 2e-6, "bf16x2": 6e-5}
+SPLIT_TOL = {"fp16x2": 1e-6, "bf16x3": 1e-6, "bf16x2": 6e-5}
 
 
 def add_form(batch, p, form, albedo=0.5):
@@ -187,7 +187,7 @@ def test_split_converges_below_1e_6(gpu, form):
                 break
         assert r[s] < 1e-6
         ref, _ = orc.source_iteration(p["I0"], p["B"], p["P"], 0.5, int(b.residuals()[1][s]), threads=16)
-        assert rel(state(b, s, p), ref) <= 2e-6
+        assert rel(state(b, s, p), ref) <= 1e-6
     p["ds"].close()
 
 
@@ -357,3 +357,32 @@ def test_batch_keeps_its_panel_through_cache_eviction(gpu, form):
     assert np.array_equal(got, want)
     p["ds"].close()
     q["ds"].close()
+
+
+def test_batch_autotune_choice_is_used_and_every_tile_agrees(gpu):
+    """rsh_batch_autotune times the grouped FP64 tiles on the device and records the
+    fastest for the class; batches created afterwards run it (slot_config), and
+    every tile gives bit-identical states (same per-output summation order).
+    Split forms: the tuner chooses between the resident- and streaming-panel
+    grouped kernels per M, and new slots take that kernel."""
+    choice, ms = rs.batch_autotune("fp64", 400, n_cells=20000, iterations=4)
+    assert choice[0] == "tile" and choice[1] in [(4, 2), (4, 4), (8, 2)] and ms > 0
+    assert rs.batch_tuned("fp64", 400) == choice
+    p = make(400, 3000, "hg", 0.6, seed=51)
+    states = {}
+    for tile in [None, (4, 2), (4, 4), (8, 2)]:
+        with rs.Batch(2) as b:
+            b.set_f64_tile(tile)
+            s = add(b, p)
+            b.run(3)
+            assert b.slot_config(s, "fp64") == ("tile", tile or choice[1])
+            states[tile] = state(b, s, p)
+    for tile in [(4, 2), (4, 4), (8, 2)]:
+        assert np.array_equal(states[tile], states[None])
+    for form in ("bf16x3", "fp16x2"):
+        kchoice, kms = rs.batch_autotune(form, 400, n_cells=20000, iterations=4)
+        assert kchoice[0] == "kernel" and kchoice[1] in ("resident", "streaming") and kms > 0
+        with rs.Batch(2) as b:
+            s = add_form(b, p, form)
+            assert b.slot_config(s, form) == kchoice
+    p["ds"].close()

@@ -190,3 +190,68 @@ def test_resident_kernel_rejects_face_state(gpu):
             ds.scatter_split("bf16x2", state, scale, 4, 64, 0, 26, out, waves=1)
         for a in (state, scale, out):
             a.free()
+
+
+def _bf16_rne(x):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32 values."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def _parts_bits(x, form):
+    """split_parts: NT parts of 16 bits, each the nearest value of the format to what is left."""
+    nt = 3 if form == "bf16x3" else 2
+    r = x.astype(np.float32)
+    out = []
+    for _ in range(nt):
+        if form == "fp16x2":
+            p = r.astype(np.float16)
+            out.append(p.view(np.uint16))
+            r = (r - p.astype(np.float32)).astype(np.float32)
+        else:
+            p = _bf16_rne(r)
+            out.append((p.view(np.uint32) >> 16).astype(np.uint16))
+            r = (r - p).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
+@pytest.mark.parametrize("faces", [4, 1])
+def test_cell_pack_matches_layout_definition(gpu, form, faces):
+    """The one-pass cell-level pack against a numpy restatement of its definition:
+    cell scale 2^-e, e = 11 - ilogb(max_{j,f} |I[j][c][f]|); value (((f0 + f1) + f2) + f3) / 4
+    (faces = 4) or I[j][c] (faces = 1), times 2^e, rounded to FP32, split into parts that
+    each round what is left to the 16-bit format; stored MFMA-native: the 16-byte vector
+    ((tile * steps + s) * parts + part) * 64 + lane holds cells 16 tile + lane % 16,
+    directions 32 s + 8 (lane / 16) + e. Ragged cells and directions pad with zeros."""
+    n, m = 1000, 200            # 62.5 cell tiles, 6.25 K-steps
+    rng = np.random.default_rng(31 + faces)
+    illum = rng.lognormal(0.0, 1.5, size=(m, n, faces))
+    illum[:, 17, :] *= 2.0 ** -40      # a cell far from the others' magnitude
+    d_illum = D.DeviceArray.from_host(illum)
+    sb, cb = rs.split_sizes(form, 1, n, m)
+    state = D.DeviceArray((sb,), np.uint8)
+    scale = D.DeviceArray((cb // 4,), np.float32)
+    if faces == 4:
+        rs.pack_split_avg_dev(form, d_illum, n, m, state, scale)
+    else:
+        rs.pack_split_dev(form, d_illum, 1, n, m, state, scale)
+    D.synchronize()
+    got_state, got_scale = state.to_host(), scale.to_host()
+    for a in (d_illum, state, scale):
+        a.free()
+    mx = np.max(np.abs(illum), axis=(0, 2))
+    e = 11 - (np.frexp(mx)[1] - 1)
+    assert np.array_equal(got_scale, np.ldexp(np.float32(1.0), -e).astype(np.float32))
+    v = (((illum[..., 0] + illum[..., 1]) + illum[..., 2]) + illum[..., 3]) * 0.25 if faces == 4 else illum[..., 0]
+    x = (v * np.ldexp(1.0, e)[None, :]).astype(np.float32)          # [m][n]
+    steps, tiles = (m + 31) // 32, (n + 15) // 16
+    xp = np.zeros((steps * 32, tiles * 16), np.float32)
+    xp[:m, :n] = x
+    parts = _parts_bits(xp, form)
+    nt = len(parts)
+    # [tile][s][part][lane = 16 g + c][e] with direction 32 s + 8 g + e, cell 16 tile + c
+    want = np.stack([p.reshape(steps, 4, 8, tiles, 16).transpose(3, 0, 1, 4, 2) for p in parts], axis=2)
+    want = want.reshape(tiles, steps, nt, 64, 8)
+    assert np.array_equal(got_state.view(np.uint16).reshape(want.shape), want)

@@ -162,10 +162,14 @@ def test_transport_iteration_matches_oracle(gpu):
             a.free()
 
 
-def test_sweep_multipass_levels_match_oracle(gpu):
+@pytest.mark.parametrize("cls", [1, 0])
+def test_sweep_multipass_levels_match_oracle(gpu, cls):
     """A mesh whose levels hold more cells than one pass (512), so that passes of
     one level read upstream values from the LDS ring of earlier passes and from
-    global memory within the level, and levels end on their last pass's ring."""
+    global memory within the level, and levels end on their last pass's ring.
+    Class 0 (the sphere-source test task) also carries its per-node source flags
+    in the out faces' ring-code fields: on this mesh many nodes lie in the source
+    and share ring words with in-face ring codes across several passes."""
     v, pairs = sw.mesh_cube_tetra(24)
     n = v.shape[0]
     dirs = np.random.default_rng(31).normal(size=(6, 3))
@@ -174,8 +178,33 @@ def test_sweep_multipass_levels_match_oracle(gpu):
     rng = np.random.default_rng(32)
     S = rng.lognormal(size=(6, n))
     old = rng.lognormal(size=(6, n, 4))
-    ref, ref_norm = orc.sweep(per, pairs, S, old, cls=1)
-    got, norm, levels = gpu_sweep(v, pairs, per, S, old, 1)
+    nodes = sw.tetra_geometry(v)[3] if cls == 0 else None
+    if cls == 0:
+        centre = np.array([0.5, 0.0, 0.0])
+        inside = np.linalg.norm(nodes.reshape(-1, 3) - centre, axis=1) < 0.05
+        assert inside.sum() > 100     # many source nodes
+    ref, ref_norm = orc.sweep(per, pairs, S, old, cls=cls, nodes=nodes)
+    got, norm, levels = gpu_sweep(v, pairs, per, S, old, cls, nodes)
     assert n / min(levels) > 512          # some level of every direction spans several passes
+    check(got, ref)
+    assert norm.max() == pytest.approx(ref_norm, rel=1e-10)
+
+
+def test_sweep_all_directions_at_196608_cells(gpu):
+    """The bench's sweep workload, every direction: 6 x 32^3 = 196,608 tetrahedra,
+    M = 400 Fibonacci directions, class 1, against the sequential oracle. Every
+    face value of every direction is compared, so the LDS-ring passes, the
+    reads of older upstream values from global memory and the store ordering of
+    faces out on both sides are all exercised at the size that is timed."""
+    v, pairs = sw.mesh_cube_tetra(32)
+    n, m = v.shape[0], 400
+    xyz, _, _ = rs.fibonacci_directions(m)
+    per = sw.prepare(v, pairs, xyz, threads=16)
+    rng = np.random.default_rng(0x5EED + 7)
+    S = rng.lognormal(size=(m, n))
+    old = np.ones((m, n, 4))
+    ref, ref_norm = orc.sweep(per, pairs, S, old, cls=1, threads=16)
+    got, norm, levels = gpu_sweep(v, pairs, per, S, old, 1)
+    assert n == 196608 and len(levels) == m and max(levels) > 100
     check(got, ref)
     assert norm.max() == pytest.approx(ref_norm, rel=1e-10)
