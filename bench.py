@@ -629,6 +629,10 @@ def main() -> int:
     pairs = n * m
     flops = 2.0 * m * m * n
     achieved_tflops = flops / (kern_ms * 1e-3) / 1e12
+    # the matrix-core rate this device reaches in a register-only loop (no memory traffic)
+    peak64 = rs.mfma_peak("fp64")
+    peak_bf16 = rs.mfma_peak("bf16")
+    log(f"[bench] register-only MFMA loops: FP64 16x16x4 {peak64:.1f} TFLOP/s, BF16 16x16x32 {peak_bf16:.0f} TFLOP/s")
     traffic = None
     if os.path.exists(PMC_SUMMARY):
         with open(PMC_SUMMARY) as f:
@@ -702,6 +706,11 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_MFMA_PEAK_TFLOPS,
             "traffic": traffic,
+            "peak_measured": peak64,
+            "frac_of_measured": achieved_tflops / peak64,
+            "peak_measured_how": "register-only v_mfma_f64_16x16x4f64 loop, 8 waves per CU, 8 independent "
+                                 "accumulators per wave (rsh_mfma_peak)",
+            "bf16_peak_measured": peak_bf16,
             "algorithmic_bytes_per_launch": pairs * 40,
             "algorithmic_flops_per_launch": flops,
         },

@@ -231,12 +231,13 @@ int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream
 /* Relative error of every result against an FP64 reference on the device:
  * max and mean over rows x n_cells of |got - ref| / |ref| (got FP64, or FP32
  * when got_f32; row r at d_got + r * ld_got and d_ref + r * ld_ref elements),
- * and how many exceed `threshold`; a NaN counts as an infinite error.
+ * how many exceed `threshold`, and (argmax, may be NULL) the flat index
+ * r * n_cells + c of the largest; a NaN counts as an infinite error.
  * Synchronous on `stream`. No reference counterpart: it states the split
  * forms' error per result over a whole output instead of on sampled cells. */
 int rsh_dev_rel_error(const void* d_got, int32_t got_f32, int64_t ld_got, const double* d_ref, int64_t ld_ref,
                       int64_t rows, int64_t n_cells, double threshold, double* max_rel, double* mean_rel,
-                      int64_t* count_above, void* stream);
+                      int64_t* count_above, int64_t* argmax, void* stream);
 /* Achievable matrix-core rate of the device, TFLOP/s, from register-only loops
  * of independent MFMAs (8 waves per CU, no memory traffic): kind 0 =
  * v_mfma_f64_16x16x4f64 (the FP64 kernels' instruction), 1 =

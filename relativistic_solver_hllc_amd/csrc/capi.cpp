@@ -647,34 +647,39 @@ int rsh_dev_copy_f64(double* dst, const double* src, int64_t count, void* stream
 
 int rsh_dev_rel_error(const void* d_got, int32_t got_f32, int64_t ld_got, const double* d_ref, int64_t ld_ref,
                       int64_t rows, int64_t n_cells, double threshold, double* max_rel, double* mean_rel,
-                      int64_t* count_above, void* stream) {
+                      int64_t* count_above, int64_t* argmax, void* stream) {
     if (!max_rel || !mean_rel || !count_above || rows < 0 || n_cells < 0 || ld_got < n_cells || ld_ref < n_cells)
         return fail(RSH_E_ARG, "invalid relative-error arguments");
     *max_rel = 0.0;
     *mean_rel = 0.0;
     *count_above = 0;
+    if (argmax) *argmax = -1;
     if (rows == 0 || n_cells == 0) return RSH_OK;
     if (!d_got || !d_ref) return fail(RSH_E_ARG, "null device pointer");
     const int gx = (int)std::min<int64_t>((n_cells + 255) / 256, 64);
     const int gy = (int)std::min<int64_t>(rows, 128);
     double* d_part = nullptr;
-    RSH_HIP_TRY(hipMalloc(&d_part, (size_t)gx * gy * 3 * sizeof(double)));
+    RSH_HIP_TRY(hipMalloc(&d_part, (size_t)gx * gy * 4 * sizeof(double)));
     hipError_t e = launch_rel_error(d_got, got_f32 ? 1 : 0, ld_got, d_ref, ld_ref, rows, n_cells, threshold, d_part,
                                     gx, gy, (hipStream_t)stream);
-    std::vector<double> part((size_t)gx * gy * 3);
+    std::vector<double> part((size_t)gx * gy * 4);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e == hipSuccess) e = hipMemcpy(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost);
     hipFree(d_part);
     if (e != hipSuccess) return hip_fail(e, "relative-error reduction");
-    double mx = 0.0, sum = 0.0, above = 0.0;
-    for (size_t b = 0; b < part.size(); b += 3) {
-        mx = std::max(mx, part[b]);
+    double mx = -1.0, sum = 0.0, above = 0.0, at = -1.0;
+    for (size_t b = 0; b < part.size(); b += 4) {
+        if (part[b] > mx) {
+            mx = part[b];
+            at = part[b + 3];
+        }
         sum += part[b + 1];
         above += part[b + 2];
     }
-    *max_rel = mx;
+    *max_rel = mx < 0.0 ? 0.0 : mx;
     *mean_rel = sum / ((double)rows * (double)n_cells);
     *count_above = (int64_t)above;
+    if (argmax) *argmax = (int64_t)at;
     return RSH_OK;
 }
 

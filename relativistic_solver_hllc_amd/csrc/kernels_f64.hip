@@ -342,14 +342,15 @@ __global__ __launch_bounds__(256) void copy_f64_kernel(v2d* __restrict__ dst, co
 __global__ void copy_tail_kernel(double* __restrict__ dst, const double* __restrict__ src) { dst[0] = src[0]; }
 
 // Relative error of every result against an FP64 reference, reduced per
-// workgroup: partial[b] = (max, sum, count above threshold) over the
-// workgroup's share of rows x n. A NaN result (or a NaN relative error) counts
-// as an infinite error. Row r of got starts at r * ld_got, of ref at r * ld_ref.
+// workgroup: partial[b] = (max, sum, count above threshold, flat index r * n + c
+// of the max) over the workgroup's share of rows x n. A NaN result (or a NaN
+// relative error) counts as an infinite error. Row r of got starts at
+// r * ld_got, of ref at r * ld_ref.
 template <typename T>
 __global__ __launch_bounds__(256) void rel_error_kernel(const T* __restrict__ got, int64_t ld_got,
                                                         const double* __restrict__ ref, int64_t ld_ref, int64_t rows,
                                                         int64_t n, double threshold, double* __restrict__ partial) {
-    double mx = 0.0, sum = 0.0, above = 0.0;
+    double mx = -1.0, sum = 0.0, above = 0.0, at = -1.0;
     for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
         const T* g = got + r * ld_got;
         const double* f = ref + r * ld_ref;
@@ -358,29 +359,39 @@ __global__ __launch_bounds__(256) void rel_error_kernel(const T* __restrict__ go
             const double d = fabs((double)g[c] - want);
             double e = d == 0.0 ? 0.0 : d / fabs(want);
             if (e != e) e = HUGE_VAL;
-            mx = fmax(mx, e);
+            if (e > mx) {
+                mx = e;
+                at = (double)(r * n + c);
+            }
             sum += e;
             above += e > threshold ? 1.0 : 0.0;
         }
     }
-    __shared__ double red[3][256];
+    __shared__ double red[4][256];
     red[0][threadIdx.x] = mx;
     red[1][threadIdx.x] = sum;
     red[2][threadIdx.x] = above;
+    red[3][threadIdx.x] = at;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
-            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + st]);
-            red[1][threadIdx.x] += red[1][threadIdx.x + st];
-            red[2][threadIdx.x] += red[2][threadIdx.x + st];
+            const int o = threadIdx.x + st;
+            if (red[0][o] > red[0][threadIdx.x] ||
+                (red[0][o] == red[0][threadIdx.x] && red[3][o] >= 0 && (red[3][threadIdx.x] < 0 || red[3][o] < red[3][threadIdx.x]))) {
+                red[0][threadIdx.x] = red[0][o];
+                red[3][threadIdx.x] = red[3][o];
+            }
+            red[1][threadIdx.x] += red[1][o];
+            red[2][threadIdx.x] += red[2][o];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double* o = partial + 3 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+        double* o = partial + 4 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x);
         o[0] = red[0][0];
         o[1] = red[1][0];
         o[2] = red[2][0];
+        o[3] = red[3][0];
     }
 }
 

@@ -444,11 +444,13 @@ def rel_error_dev(got, ref, rows: int, n_cells: int, ld_got: Optional[int] = Non
     """Relative error |got - ref| / |ref| of every one of rows x n_cells results (got FP64 or FP32,
     ref FP64, both device arrays) reduced on the device: max, mean and the count above threshold."""
     got_f32 = 1 if np.dtype(got.dtype) == np.float32 else 0
-    mx, mean, above = C.c_double(), C.c_double(), C.c_int64()
+    mx, mean, above, at = C.c_double(), C.c_double(), C.c_int64(), C.c_int64()
     N.call("rsh_dev_rel_error", _ptr(got), got_f32, int(ld_got or n_cells), _ptr(ref), int(ld_ref or n_cells),
-           int(rows), int(n_cells), float(threshold), C.byref(mx), C.byref(mean), C.byref(above), _stream(stream))
+           int(rows), int(n_cells), float(threshold), C.byref(mx), C.byref(mean), C.byref(above), C.byref(at),
+           _stream(stream))
+    worst = (at.value // int(n_cells), at.value % int(n_cells)) if at.value >= 0 else None
     return {"max_rel_err": mx.value, "mean_rel_err": mean.value, "count_above": above.value,
-            "threshold": threshold, "results": int(rows) * int(n_cells)}
+            "threshold": threshold, "results": int(rows) * int(n_cells), "worst_row_cell": worst}
 
 
 # ------------------------------------------------------------------ problem (InitDevice ...)

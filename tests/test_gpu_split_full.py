@@ -53,8 +53,9 @@ def test_every_result_of_the_full_workload(workload, form, out_f32):
     e = rs.rel_error_dev(out, ref, M, N, threshold=TOL[form][0])
     for a in (state, scale, out):
         a.free()
-    print(f"{form} {'fp32' if out_f32 else 'fp64'} out: max {e['max_rel_err']:.3e}, mean {e['mean_rel_err']:.3e}, "
-          f"above {TOL[form][0]:g}: {e['count_above']} of {e['results']}")
+    print(f"{form} {'fp32' if out_f32 else 'fp64'} out: max {e['max_rel_err']:.3e} at (row, cell) "
+          f"{e['worst_row_cell']}, mean {e['mean_rel_err']:.3e}, above {TOL[form][0]:g}: {e['count_above']} of "
+          f"{e['results']}")
     assert e["results"] == M * N
     assert e["max_rel_err"] <= TOL[form][0] and e["mean_rel_err"] <= TOL[form][1]
 
@@ -72,7 +73,7 @@ def test_rel_error_reduction_counts_planted_errors(gpu):
     ref, got = D.DeviceArray.from_host(ref_h), D.DeviceArray.from_host(got_h)
     e = rs.rel_error_dev(got, ref, m, n, threshold=1e-6)
     want = np.abs(got_h - ref_h) / ref_h
-    assert e["max_rel_err"] == want.max() and e["count_above"] == 2
+    assert e["max_rel_err"] == want.max() and e["count_above"] == 2 and e["worst_row_cell"] == (3, 17)
     assert e["mean_rel_err"] == pytest.approx(want.mean(), rel=1e-12)
     # a NaN counts as an infinite error
     got_h[0, 0] = np.nan
