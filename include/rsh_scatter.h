@@ -197,6 +197,21 @@ int rsh_scatter_split_dev(rsh_dirset ds, int32_t form, const void* d_state, cons
                           int32_t faces, int64_t n_cells, int32_t k0, int32_t k1, void* d_out, int64_t ld_out,
                           int32_t out_f32, void* stream, int32_t waves);
 
+/* Allocation-free variant of rsh_scatter_f64_dev (MFMA kernel, tuned or
+ * default tile). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Instead of the direction set's
+ * panel cache it builds the phase panel of [k0, k1) into a fixed per-device
+ * scratch array of rsh_noalloc_scratch_bytes() bytes (8 MiB: M up to about
+ * 1000 directions; larger panels return RSH_E_UNSUPPORTED) and calls no
+ * hipMalloc / hipFree. Results are bit-identical to rsh_scatter_f64_dev's.
+ * Every call on a device rewrites that device's scratch: calls that overlap on
+ * one device (other streams or host threads) race and must be ordered by the
+ * caller. */
+int rsh_scatter_f64_dev_noalloc(rsh_dirset ds, const double* d_illum, int32_t faces, int64_t n_cells,
+                                int32_t k0, int32_t k1, double* d_out, int64_t ld_out, void* stream);
+int64_t rsh_noalloc_scratch_bytes(void);
+
 /* Seeded log-normal fill exp(mu + sigma * N(0,1)) on the device (counter-based
  * generator: the value at index i depends only on (seed, i)). */
 int rsh_fill_lognormal_dev(double* d_out, int64_t count, double mu, double sigma, uint64_t seed,

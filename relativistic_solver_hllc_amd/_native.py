@@ -121,6 +121,8 @@ PROTOTYPES = {
     "rsh_dev_scale_f64": (C.c_int, [vp, i64, f64, vp]),
     "rsh_dev_copy_f64": (C.c_int, [vp, vp, i64, vp]),
     "rsh_mfma_peak": (C.c_int, [i32, i32, P_f64]),
+    "rsh_scatter_f64_dev_noalloc": (C.c_int, [vp, vp, i32, i64, i32, i32, vp, i64, vp]),
+    "rsh_noalloc_scratch_bytes": (i64, []),
     "rsh_dev_rel_error": (C.c_int, [vp, i32, i64, vp, i64, i64, i64, f64, P_f64, P_f64, P_i64, P_i64, vp]),
     "rsh_dev_synchronize": (C.c_int, [i32]),
     "rsh_dev_mem_info": (C.c_int, [i32, C.POINTER(i64), C.POINTER(i64)]),

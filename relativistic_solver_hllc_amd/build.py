@@ -30,7 +30,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{REPO_DIR}/include", f"-I{CSRC}"]
 
 HIP_SOURCES = ["kernels_f64.hip", "kernels_split.hip", "kernels_iter.hip", "kernels_iter_split.hip",
-               "kernels_moments.hip", "kernels_split_resident.hip", "kernels_sweep.hip", "kernels_peak.hip"]
+               "kernels_moments.hip", "kernels_split_resident.hip", "kernels_sweep.hip", "kernels_peak.hip", "noalloc.hip"]
 HOST_SOURCES = ["capi.cpp", "batch.cpp", "server.cpp", "sweep.cpp", "sweep_prep.cpp"]
 
 # Drop-in shim of the reference's cuda_solve.h names (stand-alone types) and the

@@ -197,6 +197,13 @@ class DirectionSet:
         N.call("rsh_scatter_f64_dev", self.handle, _ptr(illum), faces, n_cells, k0, k1, _ptr(out),
                n_cells if ld is None else ld, _stream(stream), F64_KERNELS[kernel], t)
 
+    def scatter_f64_noalloc(self, illum, faces: int, n_cells: int, k0: int, k1: int, out,
+                            ld: Optional[int] = None, stream=None) -> None:
+        """rsh_scatter_f64_dev_noalloc: the MFMA path with the phase panel built in the library's fixed
+        per-device scratch (no device allocation; overlapping calls on one device race)."""
+        N.call("rsh_scatter_f64_dev_noalloc", self.handle, _ptr(illum), faces, n_cells, k0, k1, _ptr(out),
+               ld or n_cells, _stream(stream))
+
     def scatter_split(self, form: str, state, cell_scale, faces: int, n_cells: int, k0: int, k1: int, out,
                       ld: Optional[int] = None, out_f32: bool = False, stream=None, waves: int = 0) -> None:
         """Split-precision S rows [k0, k1) from a packed state (see pack_split_dev)."""
