@@ -466,7 +466,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         // resident-panel grouped kernel where its plan fits and is specialised
         SplitPlan rplan{};
         int sets = 0;
-        if (plan_split_resident(form, ds->m, ds->m, device_cus(b->device), &rplan, &sets) &&
+        if (plan_split_resident(form, ds->m, ds->m, device_cus(b->device), &rplan, &sets, true) &&
             iter_split_res_supported(form, rplan.rb_per_pass, rplan.steps)) {
             s.splan = rplan;
             s.res = true;

@@ -84,7 +84,9 @@ int build_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t 
 int get_panel(rsh_dirset ds, int k0, int k1, int faces, int form, hipStream_t st, PanelRef* out,
               const SplitPlan* resident, int f64_max_rb) {
     std::lock_guard<std::mutex> lock(ds->mu);
-    auto key = std::make_tuple(k0, k1, faces, form, form == RSH_FORM_FP64 ? f64_max_rb : (resident ? 1 : 0));
+    // layout: FP64 -> pass cap; split -> 0 streaming plan, 100 + row blocks per group for a resident plan
+    auto key = std::make_tuple(k0, k1, faces, form,
+                               form == RSH_FORM_FP64 ? f64_max_rb : (resident ? 100 + resident->rb_per_pass : 0));
     auto it = ds->panels.find(key);
     if (it != ds->panels.end()) {
         // Make st wait for the build only while it may still be running.
@@ -222,7 +224,7 @@ int scatter_split(rsh_dirset ds, int form, const void* d_state, const float* d_c
     SplitPlan rplan{};
     int sets = 0;
     const bool resident_ok = faces == 1 && ld < kSplitResidentMaxLd &&
-                             plan_split_resident(form, ds->m, k1 - k0, device_cus(ds->device), &rplan, &sets);
+                             plan_split_resident(form, ds->m, k1 - k0, device_cus(ds->device), &rplan, &sets, false);
     if (waves == 1 && !resident_ok)
         return fail(RSH_E_UNSUPPORTED, "resident-panel kernel needs cell-level state (faces = 1), ld_out below 2^24 "
                                        "and a panel slice that fits LDS with at most one row group per CU of an XCD");

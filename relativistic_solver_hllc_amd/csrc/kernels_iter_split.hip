@@ -3,7 +3,7 @@
 // FP64 grouped kernel (kernels_iter.hip).
 //
 // Epilogue, per output (k, c): S in FP64 from the main and cross FP32
-// accumulators and the exact scales; I_new = (1 - a) B + a S in FP64; the old
+// sums and the exact scales; I_new = (1 - a) B + a S in FP64; the old
 // value is the state the MFMA consumed (parts summed, rescaled exactly); the
 // residual |I_old - I_new| / I_new is FP64; I_new is re-split into the packed
 // layout of the other ping-pong buffer.
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 }
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const double S = ((double)accm[rb][r] + (double)accx[rb][r]) * (double)ri[r] * cinv;
+                    const double S = split_value(accm[rb][r], accx[rb][r]) * (double)ri[r] * cinv;
                     const double nv = oma * bv[r] + alb * S;
                     float of = 0.0f;
 #pragma unroll

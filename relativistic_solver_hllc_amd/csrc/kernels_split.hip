@@ -294,7 +294,7 @@ __global__ __launch_bounds__(64 * W) void split_mfma_kernel(const v4u* __restric
         for (int r = 0; r < 4; r++) {
             const int row = (rb0 + rb) * 16 + 4 * g + r;
             if (row < rows) {
-                const double v = ((double)accm[i][r] + (double)accx[i][r]) * (double)row_inv[row] * cinv;
+                const double v = split_value(accm[i][r], accx[i][r]) * (double)row_inv[row] * cinv;
                 if (out_f32)
                     __builtin_nontemporal_store((float)v, static_cast<float*>(out) + (int64_t)row * ld + cell);
                 else

@@ -7,8 +7,12 @@
 // 100 KB of LDS per workgroup only one workgroup fits a CU, so those barriers
 // are exposed. Here each workgroup owns a fixed group of RBW row blocks (16 RBW
 // output directions) and copies that slice of the panel for ALL K-steps into
-// LDS once (RBW * steps * NT KB, <= 160 KB), then streams cells through it with
-// no further barrier: persistent workgroups, one per CU.
+// LDS once, then streams cells through it with no further barrier: persistent
+// workgroups, one per CU. Two parts per row block are resident (hi and lo;
+// hi and mid for the three-part form, whose lo part is read per K-step from
+// L2, where every row group's slice of it stays): 2 KB per (K-step, row block),
+// so five row blocks fit at M = 400 (130 KB) and the 25 row blocks of M = 400
+// split into five groups without padding.
 //
 // Work split. The G row groups of a problem are served by G workgroups that
 // sit on the same XCD (workgroup b runs on XCD b % 8) and walk the same
@@ -17,23 +21,28 @@
 // chunks; within it, the floor(CUs per XCD / G) sets of G workgroups take
 // chunks round-robin.
 //
-// Per wave: two 16-cell tiles x RBW row blocks, so every A fragment read from
-// LDS feeds two MFMAs per product; MFMAs are issued product-major so that no
-// MFMA depends on the one before it; state vectors are prefetched several
-// K-steps ahead, across chunk boundaries too, with ordinary (L2-allocating)
-// loads, since the other row groups of the XCD read the same vectors. Products
-// and their order per (row block, tile) are those of the streaming kernel (main hi*hi; cross hi*lo + lo*hi, and for three
-// parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi), so the kernels give
-// bit-identical results on the same state and panel.
+// Per wave (four waves, one per SIMD): four 16-cell tiles x RBW row blocks, so
+// every A fragment read from LDS feeds four MFMAs per product; MFMAs are issued
+// product-major so that no MFMA depends on the one before it; state vectors
+// are prefetched K-steps ahead, across chunk boundaries too, with ordinary
+// (L2-allocating) loads, since the other row groups of the XCD read the same
+// vectors. Products and their order per (row block, tile) are those of the
+// streaming kernel (each K-step's main hi*hi formed alone and added in FP32;
+// cross hi*lo + lo*hi, and for three parts hi*lo, hi*mid, mid*mid, mid*hi,
+// lo*hi), so the kernels give bit-identical results on the same state and
+// panel.
 #include "mfma_split_core.hpp"
 
 namespace rsh {
 
 namespace {
 
-constexpr int kResWaves = 8;   // waves per workgroup (2 per SIMD)
-constexpr int kResTiles = 2;   // 16-cell tiles per wave
+constexpr int kResWaves = 8;   // waves per workgroup of the grouped iteration kernel (2 per SIMD)
+constexpr int kResTiles = 2;   // 16-cell tiles per wave of the grouped iteration kernel
 constexpr int kResChunkTiles = kResWaves * kResTiles;  // 256 cells per chunk
+constexpr int kRes4Waves = 4;  // waves per workgroup of the single-problem kernel (1 per SIMD)
+constexpr int kRes4Tiles = 4;  // 16-cell tiles per wave of the single-problem kernel
+static_assert(kRes4Waves * kRes4Tiles == kResChunkTiles, "both resident kernels walk 256-cell chunks");
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kResLdsBytes = 160 * 1024;
 constexpr int kIterResSub = 4;  // 256-cell chunks per item of the grouped iteration kernel
@@ -44,12 +53,12 @@ constexpr int kBufWord3 = 0x00020000;
 // and not read again by this kernel).
 constexpr int kNontemporal = 2;
 
-// State prefetch depth in K-steps (registers: XD x tiles x parts x 4 VGPRs).
-// Measured at 1.57 M cells x 400 directions: two parts 2 / 4 / 6 steps ->
-// 2.21 / 2.15 / 1.99 ms with FP64 output; three parts 2 / 3 / 4 -> 3.29 /
-// 3.25 / 3.35 ms.
+// State prefetch depth in K-steps of the single-problem kernel (registers:
+// XD x 4 tiles x parts x 4 VGPRs); the three-part form also prefetches the lo
+// part of its panel slice kResLoXd K-steps ahead (RBW x 4 VGPRs each).
 template <int NT>
-constexpr int res_xd() { return NT == 2 ? 6 : 3; }
+constexpr int res_xd() { return NT == 2 ? 3 : 2; }
+constexpr int kResLoXd = 2;
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -59,42 +68,63 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-// One K-step of MFMAs for kResTiles tiles x RBW row blocks: A fragments from
-// the LDS panel (a_base: this lane's vector of the step's first fragment), B
-// from the state vectors in bx. A fragments of one part at a time (part-major),
-// so that only RBW fragments are live. Product-major order: each round touches
-// every (row block, tile) accumulator once, so an MFMA never waits for the
-// previous result of its own accumulator. Products and their order per
-// accumulator are those of the streaming kernel (main hi*hi; cross hi*lo +
-// lo*hi, for three parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi).
-template <int RBW, int FMT, int NT>
-__device__ __forceinline__ void res_mfma_step(const v4u* a_base, const typename Fmt<FMT>::V (&bx)[kResTiles][NT],
-                                              v4f (&accm)[RBW][kResTiles], v4f (&accx)[RBW][kResTiles]) {
+// One K-step of MFMAs for TT tiles x RBW row blocks: A fragments of the parts
+// resident in LDS from a_base (this lane's vector of the step's first
+// fragment; NL parts per row block), the lo part of a three-part panel from
+// alo when NL < NT, B from the state vectors in bx. A fragments of one part at
+// a time (part-major), so that only RBW fragments are live. Product-major
+// order: each round touches every (row block, tile) accumulator once, so an
+// MFMA never waits for the previous result of its own accumulator. The main
+// product hi*hi of the step is formed in fresh accumulators, half of the tiles
+// at a time (few live temporaries), and added to accm with main_add. Products
+// and their order per accumulator are those of the streaming kernel (cross:
+// hi*lo + lo*hi; for three parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi).
+template <int RBW, int FMT, int NT, int TT, int NL>
+__device__ __forceinline__ void res_mfma_step(const v4u* a_base, const typename Fmt<FMT>::V (&bx)[TT][NT],
+                                              const typename Fmt<FMT>::V* alo, v4f (&accm)[RBW][TT],
+                                              v4f (&accx)[RBW][TT]) {
     typedef typename Fmt<FMT>::V V;
     V a[RBW];
     auto fetch = [&](int p) {
 #pragma unroll
-        for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NT + p) * 64]);
+        for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NL + p) * 64]);
     };
-    auto round = [&](v4f (&acc)[RBW][kResTiles], int pb) {
+    auto round = [&](v4f (&acc)[RBW][TT], int pb) {
 #pragma unroll
         for (int i = 0; i < RBW; i++)
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
+            for (int q = 0; q < TT; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
+    };
+    auto main_round = [&]() {  // a holds the hi fragments; row blocks chained (order_after)
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            order_after(a[i], accm[i > 0 ? i - 1 : RBW - 1]);
+            v4f ms[TT];
+#pragma unroll
+            for (int q = 0; q < TT; q++) ms[q] = Fmt<FMT>::mfma(a[i], bx[q][0], v4f{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+            for (int q = 0; q < TT; q++) accm[i][q] = main_add(accm[i][q], ms[q]);
+        }
     };
     fetch(0);
-    round(accm, 0);
     if constexpr (NT == 2) {
         round(accx, 1);
+        main_round();
         fetch(1);
         round(accx, 0);
     } else {
         round(accx, 2);
         round(accx, 1);
+        main_round();
         fetch(1);
         round(accx, 1);
         round(accx, 0);
-        fetch(2);
+        if constexpr (NL == 3) {
+            fetch(2);
+        } else {
+#pragma unroll
+            for (int i = 0; i < RBW; i++) a[i] = alo[i];
+        }
         round(accx, 0);
     }
 }
@@ -102,15 +132,20 @@ __device__ __forceinline__ void res_mfma_step(const v4u* a_base, const typename 
 // STEPS > 0: the K-step count is a compile-time constant and the K loop is
 // fully unrolled, so the state prefetch crosses no loop back-edge (where the
 // compiler would otherwise wait for every outstanding load). STEPS = 0: any M.
+// NL = parts of the panel slice resident in LDS (2); a three-part panel's lo
+// part comes from global memory (L2), kResLoXd K-steps ahead.
 template <int RBW, int FMT, int NT, int STEPS>
-__global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
+__global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
     const v4u* __restrict__ state, int64_t n, int steps_rt, const float* __restrict__ cell_inv,
     const v4u* __restrict__ panel, const float* __restrict__ row_inv, void* __restrict__ out, int64_t ld, int rows,
     int out_f32, int groups, int sets, int64_t chunks) {
     typedef typename Fmt<FMT>::V V;
+    constexpr int TT = kRes4Tiles;
+    constexpr int NL = 2;
     constexpr int XD = res_xd<NT>();
+    constexpr int XL = NT == 3 ? kResLoXd : 1;
     const int steps = STEPS > 0 ? STEPS : steps_rt;
-    extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then the row group's 16 RBW row scales
+    extern __shared__ v4u lds[];  // [steps][RBW][NL][64], then the row group's 16 RBW row scales
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -120,18 +155,38 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     const int rg = local % groups;
     if (set >= sets) return;  // whole workgroup leaves together: no barrier below is skipped by part of it
 
-    // 1. the row group's panel slice, all K-steps, into LDS (contiguous in the panel)
-    const int64_t units = (int64_t)steps * RBW * NT * 64;
-    const v4u* __restrict__ src = panel + (int64_t)rg * units;
-    for (int64_t u = threadIdx.x; u < units; u += 64 * kResWaves) lds[u] = src[u];
+    // 1. the row group's panel slice, all K-steps, parts 0 .. NL - 1, into LDS
+    const int64_t slice = (int64_t)steps * RBW * NT * 64;  // the row group's panel, 16-byte vectors
+    const v4u* __restrict__ src = panel + (int64_t)rg * slice;
+    const int64_t units = (int64_t)steps * RBW * NL * 64;
+    for (int64_t u = threadIdx.x; u < units; u += 64 * kRes4Waves) {
+        const int64_t f = u >> 6;  // (s * RBW + rb) * NL + p
+        lds[u] = src[((f / NL) * NT + f % NL) * 64 + (u & 63)];
+    }
     const int row0 = rg * RBW * 16;
     // Row scales once per workgroup into LDS: scale loads from global memory in
-    // every epilogue measurably broke the row groups' L2 sharing of the state
-    // (2.4x the state bytes fetched from HBM).
+    // every epilogue measurably broke the row groups' L2 sharing of the state.
     float* rscale = reinterpret_cast<float*>(lds + units);
-    for (int r = threadIdx.x; r < RBW * 16; r += 64 * kResWaves)
+    for (int r = threadIdx.x; r < RBW * 16; r += 64 * kRes4Waves)
         rscale[r] = row0 + r < rows ? row_inv[row0 + r] : 0.0f;
     __syncthreads();
+
+    // The lo part of a three-part panel slice through one buffer descriptor; an
+    // opaque zero in its offset, renewed per chunk, keeps the compiler from
+    // hoisting all K-steps' loads out of the chunk loop (they would not fit).
+    const __amdgpu_buffer_rsrc_t lo_rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(src), (short)0, (int)(slice * 16), kBufWord3);
+    int lo_zero = 0;
+    v4u plo[XL][RBW];
+    auto load_lo = [&](auto slot_tag, int t) {
+        if constexpr (NT == 3) {
+            constexpr int sl = decltype(slot_tag)::value;
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
+                plo[sl][i] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(
+                                                         lo_rs, lane * 16 + lo_zero, ((t * RBW + i) * NT + 2) * 1024, 0));
+        }
+    };
 
     // 2. chunks of this XCD, round-robin over its sets
     const int64_t c_begin = chunks * xcd / kXcds, c_end = chunks * (xcd + 1) / kXcds;
@@ -144,35 +199,39 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
     // offset instead of a 64-bit address per K-step. Tiles past the end of the
     // state re-read the chunk's last tile (in range) and are not written.
     __amdgpu_buffer_rsrc_t xs;
-    int xoff[kResTiles];
+    int xoff[TT];
     auto bind_chunk = [&](int64_t chunk) {
         const int64_t ct_chunk = chunk * kResChunkTiles;
         const int ntile = (int)(tiles - ct_chunk < kResChunkTiles ? tiles - ct_chunk : kResChunkTiles);
         xs = __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(state + ct_chunk * tile_units), (short)0,
                                                ntile * tile_bytes, kBufWord3);
 #pragma unroll
-        for (int q = 0; q < kResTiles; q++) {
-            const int tile = wave * kResTiles + q;
+        for (int q = 0; q < TT; q++) {
+            const int tile = wave * TT + q;
             xoff[q] = (tile < ntile ? tile : ntile - 1) * tile_bytes + lane * 16;
         }
     };
-    v4u xr[XD][kResTiles][NT];
+    v4u xr[XD][TT][NT];
     auto load_x = [&](auto slot_tag, int t) {
         constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
-        for (int q = 0; q < kResTiles; q++)
+        for (int q = 0; q < TT; q++)
 #pragma unroll
             for (int p = 0; p < NT; p++)
                 xr[sl][q][p] = __builtin_bit_cast(
                     v4u, __builtin_amdgcn_raw_buffer_load_b128(xs, xoff[q] + p * 1024, t * NT * 1024, 0));
     };
-    // First XD K-steps of a chunk. For every chunk after the first they are
+    // First K-steps of a chunk. For every chunk after the first they are
     // issued before the previous chunk's result stores: loads and stores retire
     // through one in-order counter, so a load issued after the stores could
     // only be waited for together with all of them.
     auto prefetch_head = [&]() {
+        asm volatile("v_mov_b32 %0, 0" : "=v"(lo_zero));
         static_for<0, XD>([&](auto u) {
             if (u.value < steps) load_x(u, u.value);
+        });
+        static_for<0, XL>([&](auto u) {
+            if (u.value < steps) load_lo(u, u.value);
         });
     };
 
@@ -182,42 +241,58 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
         prefetch_head();
     }
     for (; chunk < c_end; chunk += sets) {
-        const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
+        const int64_t ct0 = chunk * kResChunkTiles + wave * TT;
         // Cell scales now, ahead of the next chunk's prefetch: a load issued
         // after those could only be waited for together with all of them.
-        float cinv[kResTiles];
+        float cinv[TT];
 #pragma unroll
-        for (int q = 0; q < kResTiles; q++) {
+        for (int q = 0; q < TT; q++) {
             const int64_t cell = (ct0 + q) * 16 + (lane & 15);
             cinv[q] = cell_inv[cell < n ? cell : n - 1];
         }
-        v4f accm[RBW][kResTiles], accx[RBW][kResTiles];
+        v4f accm[RBW][TT], accx[RBW][TT];
 #pragma unroll
         for (int i = 0; i < RBW; i++)
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++) {
+            for (int q = 0; q < TT; q++) {
                 accm[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
                 accx[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
             }
-        auto kstep = [&](auto slot_tag, const int t) {
+        // K-step t with ring slot SL = t % XD and lo slot SLL = t % XL
+        auto kstep = [&](auto slot_tag, auto lo_tag, const int t) {
             constexpr int sl = decltype(slot_tag)::value;
+            constexpr int sll = decltype(lo_tag)::value;
             // Scheduling fence per K-step: in the unrolled loop the scheduler
             // would otherwise hoist every K-step's loads and run out of registers.
             __builtin_amdgcn_sched_barrier(0);
-            V bx[kResTiles][NT];
+            V bx[TT][NT];
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++)
+            for (int q = 0; q < TT; q++)
 #pragma unroll
                 for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
+            V alo[RBW];
+            if constexpr (NT == 3) {
+#pragma unroll
+                for (int i = 0; i < RBW; i++) alo[i] = __builtin_bit_cast(V, plo[sll][i]);
+                if (t + XL < steps) load_lo(lo_tag, t + XL);
+            }
             if (t + XD < steps) load_x(slot_tag, t + XD);
-            res_mfma_step<RBW, FMT, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, accm, accx);
+            res_mfma_step<RBW, FMT, NT, TT, NL>(lds + (int64_t)t * RBW * NL * 64 + lane, bx, alo, accm, accx);
         };
         if constexpr (STEPS > 0) {
-            static_for<0, STEPS>([&](auto t) { kstep(std::integral_constant<int, t.value % XD>{}, t.value); });
+            static_for<0, STEPS>([&](auto t) {
+                kstep(std::integral_constant<int, t.value % XD>{}, std::integral_constant<int, t.value % XL>{},
+                      t.value);
+            });
         } else {
-            for (int t = 0; t < steps; t += XD)
-                static_for<0, XD>([&](auto u) {
-                    if (t + u.value < steps) kstep(u, t + u.value);
+            // XD and XL divide 6: blocks of six K-steps keep every ring slot a
+            // compile-time constant
+            static_assert(6 % XD == 0 && 6 % XL == 0, "ring depths must divide the unroll of six");
+            for (int t = 0; t < steps; t += 6)
+                static_for<0, 6>([&](auto u) {
+                    if (t + u.value < steps)
+                        kstep(std::integral_constant<int, u.value % XD>{},
+                              std::integral_constant<int, u.value % XL>{}, t + u.value);
                 });
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -227,30 +302,22 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
         }
         __builtin_amdgcn_sched_barrier(0);
 
-        // Epilogue: FP64 combination of main and cross terms, exact scale removal.
-        // Row scales are re-read from LDS per chunk: an opaque zero offset keeps
-        // the compiler from hoisting all 4 RBW of them out of the chunk loop,
-        // where they would be spilled. Stores go through one buffer descriptor
-        // per row block (uniform base, row offset r * ld in a scalar register,
-        // one VGPR offset per tile), again to keep 64-bit addresses out of the
-        // vector registers.
+        // Epilogue: FP64 combination of the main and cross terms, exact scale
+        // removal. Row scales are re-read from LDS per chunk: an opaque zero
+        // offset keeps the compiler from hoisting them out of the chunk loop.
+        // Stores go through one buffer descriptor per row block (uniform base,
+        // row offset r * ld in a scalar register, one VGPR offset per tile).
         int zoff;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
         const float* __restrict__ rs = rscale + 4 * g + zoff;
-        v4f rsv[RBW];  // this lane's 4 row scales of each row block, read together
-#pragma unroll
-        for (int i = 0; i < RBW; i++) rsv[i] = *reinterpret_cast<const v4f*>(rs + i * 16);
-        double cd[kResTiles];
-#pragma unroll
-        for (int q = 0; q < kResTiles; q++) cd[q] = (double)cinv[q];
         auto store_all = [&](auto elem_tag, auto full_tag) {
             typedef decltype(elem_tag) OT;
             constexpr bool kFull = decltype(full_tag)::value;
             const int esz = (int)sizeof(OT);
-            int voff[kResTiles];
-            bool ok[kResTiles];
+            int voff[TT];
+            bool ok[TT];
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++) {
+            for (int q = 0; q < TT; q++) {
                 const int64_t cell = (ct0 + q) * 16 + (lane & 15);
                 ok[q] = cell < n;
                 voff[q] = (int)((4 * g * ld + cell) * esz);
@@ -261,13 +328,15 @@ __global__ __launch_bounds__(64 * kResWaves) void split_resident_kernel(
                 if (!kFull && rb_row >= rows) break;
                 const __amdgpu_buffer_rsrc_t os = __builtin_amdgcn_make_buffer_rsrc(
                     static_cast<OT*>(out) + (int64_t)rb_row * ld, (short)0, (int)(16 * ld * esz), kBufWord3);
+                const v4f rsv = *reinterpret_cast<const v4f*>(rs + i * 16);  // this lane's 4 row scales
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const double rsc = (double)rsv[i][r];
+                    const double rsc = (double)rsv[r];
 #pragma unroll
-                    for (int q = 0; q < kResTiles; q++) {
+                    for (int q = 0; q < TT; q++) {
                         if (!ok[q] || (!kFull && rb_row + 4 * g + r >= rows)) continue;
-                        const double v = ((double)accm[i][q][r] + (double)accx[i][q][r]) * rsc * cd[q];
+                        const double v =
+                            split_value(accm[i][q][r], accx[i][q][r]) * rsc * (double)cinv[q];
                         if constexpr (sizeof(OT) == 4)
                             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), os,
                                                                   voff[q], (int)(r * ld * esz), kNontemporal);
@@ -297,7 +366,7 @@ hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* s
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kResWaves), lds_bytes, st, state, n, steps, cell_inv, panel,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kRes4Waves), lds_bytes, st, state, n, steps, cell_inv, panel,
                        row_inv, out, ld, rows, out_f32, groups, sets, chunks);
     return hipGetLastError();
 }
@@ -419,7 +488,8 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
 #pragma unroll
                     for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
                 if constexpr (t + XD < STEPS) load_x(std::integral_constant<int, sl>{}, t + XD);
-                res_mfma_step<RBW, FMT, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, accm, accx);
+                res_mfma_step<RBW, FMT, NT, kResTiles, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, nullptr, accm,
+                                                           accx);
             });
             // This chunk's state (old values) and the output chunk through buffer
             // descriptors (uniform base, 32-bit offsets), kept before the next
@@ -478,7 +548,7 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
                     }
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const double S = ((double)accm[i][q][r] + (double)accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
+                        const double S = split_value(accm[i][q][r], accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
                         const double nv = oma * bv[r] + alb * S;
                         float of = 0.0f;
 #pragma unroll
@@ -506,24 +576,31 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
     }
 }
 
-// Largest row-block count per workgroup that the compiler keeps in 256
-// registers without spills (two parts: 5; three parts: 4).
+// Largest row-block count per workgroup: the grouped iteration kernel (eight
+// waves, two per SIMD) keeps two parts in 256 registers up to five row blocks,
+// three parts up to four; the single-problem kernel (one wave per SIMD, 512
+// registers) holds five.
 constexpr int res_max_rbw(int nt) { return nt == 2 ? 5 : 4; }
-constexpr int kResMaxRbw = 5;
+constexpr int kRes4MaxRbw = 5;
+constexpr int kRes4LdsParts = 2;  // panel parts resident in LDS in the single-problem kernel
 
 }  // namespace
 
-// Row-group plan of the resident-panel kernel for output rows [0, rows) of an
+// Row-group plan of a resident-panel kernel for output rows [0, rows) of an
 // M-direction set: rb_per_pass = RBW row blocks per workgroup (largest slice
-// that fits LDS, then balanced), passes = G row groups. False if the kernel
-// does not apply (G row groups do not fit one XCD's CUs).
-bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets) {
+// that fits LDS, then balanced), passes = G row groups. grouped: the plan of
+// the grouped iteration kernel (all parts of the slice in LDS), else that of
+// the single-problem kernel (two parts in LDS). False if the kernel does not
+// apply (G row groups do not fit one XCD's CUs).
+bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets, bool grouped) {
     const int nt = split_terms(form);
     const int steps = (m + 31) / 32;
     const int rb_need = (rows + 15) / 16;
-    const int per_rb = steps * nt * 1024 + 16 * 4;  // panel fragments + row scales
+    const int lds_parts = grouped ? nt : kRes4LdsParts;
+    const int per_rb = steps * lds_parts * 1024 + 16 * 4;  // panel fragments + row scales
     int rbw_max = kResLdsBytes / per_rb;
-    if (rbw_max > res_max_rbw(nt)) rbw_max = res_max_rbw(nt);
+    const int cap = grouped ? res_max_rbw(nt) : kRes4MaxRbw;
+    if (rbw_max > cap) rbw_max = cap;
     if (rbw_max < 1 || rb_need < 1) return false;
     const int groups = (rb_need + rbw_max - 1) / rbw_max;
     const int rbw = (rb_need + groups - 1) / groups;
@@ -542,8 +619,8 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
                                  void* d_out, int64_t ld, int out_f32, hipStream_t st) {
     if (n <= 0 || rows <= 0) return hipSuccess;
     const int nt = split_terms(form), fmt = split_fmt(form);
-    const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * nt * 1024 + (size_t)pl.rb_per_pass * 16 * 4;
-    if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > res_max_rbw(nt) || sets < 1 || ld >= kSplitResidentMaxLd)
+    const size_t lds_bytes = (size_t)pl.steps * pl.rb_per_pass * kRes4LdsParts * 1024 + (size_t)pl.rb_per_pass * 16 * 4;
+    if (lds_bytes > (size_t)kResLdsBytes || pl.rb_per_pass > kRes4MaxRbw || sets < 1 || ld >= kSplitResidentMaxLd)
         return hipErrorInvalidValue;
     const int64_t tiles = (n + 15) / 16;
     const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
@@ -554,22 +631,18 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
     return launch_res<RBW, FMT, NT, S>(blocks, lds_bytes, st, state, n, pl.steps, d_cell_inv, panel, d_row_inv, \
                                        d_out, ld, rows, out_f32, pl.passes, sets, chunks)
     // Unrolled instances for the plans of M = 200, 400, 800 (7, 13, 25 K-steps).
-#define RSH_RES(RBW, FMT, NT)                                                                          \
-    {                                                                                                  \
-        if constexpr (NT == 2 && RBW == 5) { if (pl.steps == 7) RSH_RES_S(RBW, FMT, NT, 7); }          \
-        if constexpr (NT == 2 && RBW == 5) { if (pl.steps == 13) RSH_RES_S(RBW, FMT, NT, 13); }        \
-        if constexpr (NT == 2 && RBW == 3) { if (pl.steps == 25) RSH_RES_S(RBW, FMT, NT, 25); }        \
-        if constexpr (NT == 3 && RBW == 4) { if (pl.steps == 7) RSH_RES_S(RBW, FMT, NT, 7); }          \
-        if constexpr (NT == 3 && RBW == 4) { if (pl.steps == 13) RSH_RES_S(RBW, FMT, NT, 13); }        \
-        if constexpr (NT == 3 && RBW == 2) { if (pl.steps == 25) RSH_RES_S(RBW, FMT, NT, 25); }        \
-        RSH_RES_S(RBW, FMT, NT, 0);                                                                    \
+#define RSH_RES(RBW, FMT, NT)                                                                   \
+    {                                                                                           \
+        if constexpr (RBW == 5) { if (pl.steps == 7) RSH_RES_S(RBW, FMT, NT, 7); }              \
+        if constexpr (RBW == 5) { if (pl.steps == 13) RSH_RES_S(RBW, FMT, NT, 13); }            \
+        if constexpr (RBW == 3) { if (pl.steps == 25) RSH_RES_S(RBW, FMT, NT, 25); }            \
+        RSH_RES_S(RBW, FMT, NT, 0);                                                             \
     }
-#define RSH_RES_FORMS(RBW)                                  \
-    case RBW:                                               \
-        if (fmt == 0 && nt == 2) RSH_RES(RBW, 0, 2);        \
-        if (fmt == 1 && nt == 2) RSH_RES(RBW, 1, 2);        \
-        if constexpr (RBW <= res_max_rbw(3)) RSH_RES(RBW, 0, 3); \
-        return hipErrorInvalidValue;
+#define RSH_RES_FORMS(RBW)                           \
+    case RBW:                                        \
+        if (fmt == 0 && nt == 2) RSH_RES(RBW, 0, 2); \
+        if (fmt == 1 && nt == 2) RSH_RES(RBW, 1, 2); \
+        RSH_RES(RBW, 0, 3);
     switch (pl.rb_per_pass) {
         RSH_RES_FORMS(1) RSH_RES_FORMS(2) RSH_RES_FORMS(3) RSH_RES_FORMS(4) RSH_RES_FORMS(5)
         default:

@@ -16,6 +16,46 @@ typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 
 constexpr int kTargetExp = 12;  // largest scaled magnitude ~ 2^12
 
+// Main-product accumulation. The MFMA adds its products to an FP32 accumulator
+// much larger than they are by truncating the products' low bits (measured on
+// gfx950: the error is one-signed, up to ~1e-6 relative after 12 K-steps when
+// the first K-step holds most of a row's weight). So each K-step's hi*hi
+// product is formed in a fresh (zero) accumulator and added to the running
+// main sum with an ordinary round-to-nearest FP32 add (main_add); the cross
+// terms, 2^-8 of the main sum, keep accumulating in the MFMA. Every split
+// kernel does the same per output (main sum over K-steps in ascending order,
+// cross products in the same order), so they stay bit-identical to each
+// other. The epilogue combines main and cross sums in FP64 (split_value).
+__device__ inline v4f main_add(v4f acc, v4f step) {
+    v4f r;
+#pragma unroll
+    for (int e = 0; e < 4; e++) r[e] = acc[e] + step[e];
+    return r;
+}
+
+__device__ inline double split_value(float m, float x) { return (double)m + (double)x; }
+
+// Makes x (an MFMA operand, unchanged) depend on the values deps[0 .. N-1]: an
+// empty asm per value that takes it and claims to rewrite x. The fresh main
+// MFMAs depend on no accumulator, so the scheduler would otherwise issue them
+// long before their adds and hold every product in registers meanwhile (in
+// fully unrolled K loops it spills); making the A operand of row block i's
+// main products depend on the sums row block i - 1 just added chains the row
+// blocks, so that one row block's products are in flight at a time.
+template <typename V, int N>
+__device__ __forceinline__ void order_after(V& x, const v4f (&deps)[N]) {
+    v4u u = __builtin_bit_cast(v4u, x);
+#pragma unroll
+    for (int q = 0; q < N; q++) asm("" : "+v"(u) : "v"(deps[q]));
+    x = __builtin_bit_cast(V, u);
+}
+template <typename V>
+__device__ __forceinline__ void order_after(V& x, const v4f& dep) {
+    v4u u = __builtin_bit_cast(v4u, x);
+    asm("" : "+v"(u) : "v"(dep));
+    x = __builtin_bit_cast(V, u);
+}
+
 template <int FMT>
 struct Fmt;
 template <>
@@ -60,8 +100,8 @@ struct SplitPrefetch {
 };
 
 // Accumulates, for the wave's cell tile ct (16 cells) and its share of the row
-// blocks of one pass, accm[i] += A_hi B_hi and accx[i] += cross terms over all
-// K-steps. With RH > 1 the RB row blocks of a tile are split over RH waves:
+// blocks of one pass, accm[i] = sum over K-steps of A_hi B_hi (each step's
+// product formed alone, then added, main_add) and accx[i] += cross terms. With RH > 1 the RB row blocks of a tile are split over RH waves:
 // wave part h owns row blocks h*RBW .. h*RBW + RBW - 1 (a block past RB re-reads
 // block RB - 1; its result is discarded by the caller). panel_pass points at the
 // pass's panel (steps * RB * NT fragments). Ends with a workgroup barrier.
@@ -153,8 +193,11 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
         if (more) load_panel(t + 1);
         if (t + XD < steps) load_x(slot_tag, t + XD);
         const v4u* a_base = lds[buf] + lane;
-        // A fragments of row block i + PREF are read while i's MFMAs run.
-        constexpr int PREF = RBW < 3 ? RBW : 3;
+        // A fragments of row block i + PREF are read while i's MFMAs run (two
+        // ahead where two waves share a SIMD and the state is averaged here:
+        // three would not fit their 256 registers).
+        constexpr int PMAX = (RH == 2 && F == 4) ? 2 : 3;
+        constexpr int PREF = RBW < PMAX ? RBW : PMAX;
         V apipe[PREF][NT];
 #define RSH_AFRAG(i, p) __builtin_bit_cast(V, a_base[((rb_first + (i) < RB ? rb_first + (i) : RB - 1) * NT + (p)) * 64])
 #pragma unroll
@@ -170,7 +213,8 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
 #pragma unroll
                 for (int p = 0; p < NT; p++) apipe[i % PREF][p] = RSH_AFRAG(i + PREF, p);
             }
-            accm[i] = Fmt<FMT>::mfma(a[0], bx[0], accm[i]);
+            order_after(a[0], accm[i > 0 ? i - 1 : RBW - 1]);
+            const v4f mstep = Fmt<FMT>::mfma(a[0], bx[0], v4f{0.f, 0.f, 0.f, 0.f});
             if constexpr (NT == 2) {
                 accx[i] = Fmt<FMT>::mfma(a[0], bx[1], accx[i]);
                 accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
@@ -183,6 +227,7 @@ __device__ __forceinline__ void split_accumulate(const v4u* __restrict__ state, 
                 accx[i] = Fmt<FMT>::mfma(a[1], bx[0], accx[i]);
                 accx[i] = Fmt<FMT>::mfma(a[2], bx[0], accx[i]);
             }
+            accm[i] = main_add(accm[i], mstep);
         }
 #undef RSH_AFRAG
         // Keep the reads PREF row blocks ahead: the scheduler would otherwise

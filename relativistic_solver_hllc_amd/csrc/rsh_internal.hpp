@@ -136,7 +136,9 @@ hipError_t launch_sweep(int cls, const SweepDirDev* d_dirs, int m, int64_t n,
 // stores address 16 output rows through one 32-bit buffer range, so the output
 // leading dimension must stay below kSplitResidentMaxLd cells.
 constexpr int64_t kSplitResidentMaxLd = int64_t(1) << 24;
-bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets);
+// grouped: the plan of the grouped iteration kernel (every panel part in LDS),
+// else that of the single-problem kernel (two parts in LDS).
+bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int* sets, bool grouped);
 hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const float* d_cell_inv,
                                  const void* d_panel, const float* d_row_inv, const SplitPlan& pl, int sets, int rows,
                                  void* d_out, int64_t ld, int out_f32, hipStream_t st);

@@ -40,7 +40,8 @@ struct rsh_dirset_s {
     double* d_P = nullptr;  // [m][m] FP64 phase matrix
     double* d_psi = nullptr;  // [m][kMomentFuncs] moment weights (full_area > 0 only)
     std::mutex mu;
-    // key: (k0, k1, faces, form, layout); layout 0 = streaming plan, 1 = resident-panel plan
+    // key: (k0, k1, faces, form, layout); layout: FP64 pass cap, split 0 = streaming plan,
+    // 100 + row blocks per group = resident-panel plan
     std::map<std::tuple<int, int, int, int, int>, std::shared_ptr<PanelEntry>> panels;
     bool ref_order() const { return phase == RSH_PHASE_THOMSON && !normalize; }
 };
