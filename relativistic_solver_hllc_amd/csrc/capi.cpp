@@ -692,7 +692,7 @@ int rsh_mfma_peak(int32_t device, int32_t kind, double* tflops) {
     if (rc) return rc;
     RSH_HIP_TRY(hipSetDevice(device));
     const int blocks = 2 * device_cus(device);   // 8 waves per CU: 2 per SIMD
-    const int iters = kind == 0 ? 2000 : 8000;   // ~1 ms per launch at the vendor peak
+    const int iters = kind == 0 ? 2048 : 8192;   // ~1 ms per launch at the vendor peak (multiple of the loop's unroll)
     const int reps = 5;
     double* sink = nullptr;
     hipStream_t st = nullptr;

@@ -5,7 +5,10 @@
 // Every wave keeps kPeakAcc independent accumulators and issues kPeakAcc MFMAs
 // per iteration, so no MFMA waits for the result of the one before it; the
 // operands never change and nothing touches memory until the accumulators are
-// written once at the end (so that the compiler keeps the work).
+// written once at the end (so that the compiler keeps the work). The loop body
+// is kPeakUnroll rounds: the compiler carries the accumulators across the loop
+// back-edge in VGPRs and copies them into and out of the AGPRs there, which
+// with a one-round body halved the issue rate.
 #include "mfma_split_core.hpp"
 
 namespace rsh {
@@ -13,6 +16,7 @@ namespace rsh {
 namespace {
 
 constexpr int kPeakAcc = 8;
+constexpr int kPeakUnroll = 32;  // rounds of kPeakAcc MFMAs per loop iteration; iters is a multiple
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -22,9 +26,11 @@ __global__ __launch_bounds__(256) void mfma_f64_peak_kernel(int iters, double se
     v4d acc[kPeakAcc];
 #pragma unroll
     for (int i = 0; i < kPeakAcc; i++) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int it = 0; it < iters; it++) {
+    for (int it = 0; it < iters; it += kPeakUnroll) {
 #pragma unroll
-        for (int i = 0; i < kPeakAcc; i++) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+        for (int u = 0; u < kPeakUnroll; u++)
+#pragma unroll
+            for (int i = 0; i < kPeakAcc; i++) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
     }
     double s = 0.0;
 #pragma unroll
@@ -43,9 +49,11 @@ __global__ __launch_bounds__(256) void mfma_bf16_peak_kernel(int iters, float se
     v4f acc[kPeakAcc];
 #pragma unroll
     for (int i = 0; i < kPeakAcc; i++) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int it = 0; it < iters; it++) {
+    for (int it = 0; it < iters; it += kPeakUnroll) {
 #pragma unroll
-        for (int i = 0; i < kPeakAcc; i++) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+        for (int u = 0; u < kPeakUnroll; u++)
+#pragma unroll
+            for (int i = 0; i < kPeakAcc; i++) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
     }
     float s = 0.f;
 #pragma unroll

@@ -49,16 +49,13 @@ constexpr int kIterResSub = 4;  // 256-cell chunks per item of the grouped itera
 // Buffer descriptor word 3 for raw (untyped) gfx9 buffer access: 32-bit data
 // format, so that the hardware does not treat the descriptor as invalid.
 constexpr int kBufWord3 = 0x00020000;
-// Cache policy of the result stores: non-temporal (each result is written once
-// and not read again by this kernel).
-constexpr int kNontemporal = 2;
 
 // State prefetch depth in K-steps of the single-problem kernel (registers:
 // XD x 4 tiles x parts x 4 VGPRs); the three-part form also prefetches the lo
 // part of its panel slice kResLoXd K-steps ahead (RBW x 4 VGPRs each).
 template <int NT>
 constexpr int res_xd() { return NT == 2 ? 3 : 2; }
-constexpr int kResLoXd = 2;
+constexpr int kResLoXd = 1;
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -75,57 +72,76 @@ __device__ __forceinline__ void static_for(F&& f) {
 // a time (part-major), so that only RBW fragments are live. Product-major
 // order: each round touches every (row block, tile) accumulator once, so an
 // MFMA never waits for the previous result of its own accumulator. The main
-// product hi*hi of the step is formed in fresh accumulators, half of the tiles
-// at a time (few live temporaries), and added to accm with main_add. Products
+// product hi*hi of the step is formed in fresh accumulators, one row block at a
+// time (main_round), and added to accm with main_add. Products
 // and their order per accumulator are those of the streaming kernel (cross:
 // hi*lo + lo*hi; for three parts hi*lo, hi*mid, mid*mid, mid*hi, lo*hi).
-template <int RBW, int FMT, int NT, int TT, int NL>
-__device__ __forceinline__ void res_mfma_step(const v4u* a_base, const typename Fmt<FMT>::V (&bx)[TT][NT],
+//
+// ahi (in/out): with AHEAD, ahi holds this step's hi fragments on entry, and
+// the hi fragments of the K-step at a_next are read into it as soon as this
+// step is done with its own (after the main products), so that the next step's
+// first MFMAs do not wait for LDS. Without AHEAD, the step reads its own hi
+// fragments first.
+template <int RBW, int FMT, int NT, int TT, int NL, bool AHEAD>
+__device__ __forceinline__ void res_mfma_step(const v4u* a_base, const v4u* a_next,
+                                              typename Fmt<FMT>::V (&ahi)[RBW],
+                                              const typename Fmt<FMT>::V (&bx)[TT][NT],
                                               const typename Fmt<FMT>::V* alo, v4f (&accm)[RBW][TT],
                                               v4f (&accx)[RBW][TT]) {
     typedef typename Fmt<FMT>::V V;
-    V a[RBW];
-    auto fetch = [&](int p) {
+    V (&a)[RBW] = ahi;
+    V am[RBW];  // second (and third) part
+    auto fetch_into = [&](V(&dst)[RBW], const v4u* base, int p) {
 #pragma unroll
-        for (int i = 0; i < RBW; i++) a[i] = __builtin_bit_cast(V, a_base[(i * NL + p) * 64]);
+        for (int i = 0; i < RBW; i++) dst[i] = __builtin_bit_cast(V, base[(i * NL + p) * 64]);
     };
-    auto round = [&](v4f (&acc)[RBW][TT], int pb) {
+    auto round = [&](const V(&af)[RBW], v4f (&acc)[RBW][TT], int pb) {
 #pragma unroll
         for (int i = 0; i < RBW; i++)
 #pragma unroll
-            for (int q = 0; q < TT; q++) acc[i][q] = Fmt<FMT>::mfma(a[i], bx[q][pb], acc[i][q]);
+            for (int q = 0; q < TT; q++) acc[i][q] = Fmt<FMT>::mfma(af[i], bx[q][pb], acc[i][q]);
     };
-    auto main_round = [&]() {  // a holds the hi fragments; row blocks chained (order_after)
+    // a holds the hi fragments. Software-pipelined over row blocks: row block
+    // i's products are issued before row block i - 1's are added (so that the
+    // adds do not wait for the MFMA just issued), and row block i's MFMAs are
+    // ordered after row block i - 2's adds (order_after; two row blocks of
+    // products in flight at most).
+    auto main_round = [&]() {
+        v4f ms[2][TT];
+        static_for<0, RBW>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            order_after(a[i], accm[(i + 2 * RBW - 2) % RBW]);
 #pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            order_after(a[i], accm[i > 0 ? i - 1 : RBW - 1]);
-            v4f ms[TT];
+            for (int q = 0; q < TT; q++) ms[i & 1][q] = Fmt<FMT>::mfma(a[i], bx[q][0], v4f{0.f, 0.f, 0.f, 0.f});
+            if constexpr (i >= 1) {
 #pragma unroll
-            for (int q = 0; q < TT; q++) ms[q] = Fmt<FMT>::mfma(a[i], bx[q][0], v4f{0.f, 0.f, 0.f, 0.f});
+                for (int q = 0; q < TT; q++) accm[i - 1][q] = main_add(accm[i - 1][q], ms[(i - 1) & 1][q]);
+            }
+        });
 #pragma unroll
-            for (int q = 0; q < TT; q++) accm[i][q] = main_add(accm[i][q], ms[q]);
-        }
+        for (int q = 0; q < TT; q++) accm[RBW - 1][q] = main_add(accm[RBW - 1][q], ms[(RBW - 1) & 1][q]);
     };
-    fetch(0);
+    if constexpr (!AHEAD) fetch_into(a, a_base, 0);
+    fetch_into(am, a_base, 1);
     if constexpr (NT == 2) {
-        round(accx, 1);
+        round(a, accx, 1);
         main_round();
-        fetch(1);
-        round(accx, 0);
+        if constexpr (AHEAD) fetch_into(a, a_next, 0);
+        round(am, accx, 0);
     } else {
-        round(accx, 2);
-        round(accx, 1);
+        round(a, accx, 2);
+        round(a, accx, 1);
         main_round();
-        fetch(1);
-        round(accx, 1);
-        round(accx, 0);
+        if constexpr (AHEAD) fetch_into(a, a_next, 0);
+        round(am, accx, 1);
+        round(am, accx, 0);
         if constexpr (NL == 3) {
-            fetch(2);
+            fetch_into(am, a_base, 2);
         } else {
 #pragma unroll
-            for (int i = 0; i < RBW; i++) a[i] = alo[i];
+            for (int i = 0; i < RBW; i++) am[i] = alo[i];
         }
-        round(accx, 0);
+        round(am, accx, 0);
     }
 }
 
@@ -147,7 +163,7 @@ __global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
     const int steps = STEPS > 0 ? STEPS : steps_rt;
     extern __shared__ v4u lds[];  // [steps][RBW][NL][64], then the row group's 16 RBW row scales
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar tile bounds
     const int g = lane >> 4;
     const int xcd = blockIdx.x % kXcds;
     const int local = blockIdx.x / kXcds;
@@ -236,6 +252,10 @@ __global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
     };
 
     int64_t chunk = c_begin + set;
+    // hi fragments of the next K-step (res_mfma_step reads them a step ahead)
+    V ahi[RBW];
+#pragma unroll
+    for (int i = 0; i < RBW; i++) ahi[i] = __builtin_bit_cast(V, lds[(i * NL) * 64 + lane]);
     if (chunk < c_end) {
         bind_chunk(chunk);
         prefetch_head();
@@ -277,7 +297,9 @@ __global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
                 if (t + XL < steps) load_lo(lo_tag, t + XL);
             }
             if (t + XD < steps) load_x(slot_tag, t + XD);
-            res_mfma_step<RBW, FMT, NT, TT, NL>(lds + (int64_t)t * RBW * NL * 64 + lane, bx, alo, accm, accx);
+            const int tn = t + 1 < steps ? t + 1 : 0;  // the next chunk starts at K-step 0
+            res_mfma_step<RBW, FMT, NT, TT, NL, true>(lds + (int64_t)t * RBW * NL * 64 + lane,
+                                                lds + (int64_t)tn * RBW * NL * 64 + lane, ahi, bx, alo, accm, accx);
         };
         if constexpr (STEPS > 0) {
             static_for<0, STEPS>([&](auto t) {
@@ -305,22 +327,30 @@ __global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
         // Epilogue: FP64 combination of the main and cross terms, exact scale
         // removal. Row scales are re-read from LDS per chunk: an opaque zero
         // offset keeps the compiler from hoisting them out of the chunk loop.
+        // Lane pairs (cells 2c, 2c + 1 of a tile) swap half of their results
+        // (swap_pair_lanes), so that a lane holds two adjacent cells of one row:
+        // the even lane rows 4g, 4g + 1, the odd lane rows 4g + 2, 4g + 3. Each
+        // store then writes two cells (16 bytes for FP64), half as many store
+        // instructions as one cell per lane: with one cell per lane a chunk's
+        // stores alone exceeded the wave's 63 outstanding memory operations.
         // Stores go through one buffer descriptor per row block (uniform base,
-        // row offset r * ld in a scalar register, one VGPR offset per tile).
+        // row offset h * ld in a scalar register, one VGPR offset per tile).
         int zoff;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
         const float* __restrict__ rs = rscale + 4 * g + zoff;
+        const int odd = lane & 1;
         auto store_all = [&](auto elem_tag, auto full_tag) {
             typedef decltype(elem_tag) OT;
             constexpr bool kFull = decltype(full_tag)::value;
             const int esz = (int)sizeof(OT);
             int voff[TT];
-            bool ok[TT];
+            bool ok1[TT], ok2[TT];
 #pragma unroll
             for (int q = 0; q < TT; q++) {
-                const int64_t cell = (ct0 + q) * 16 + (lane & 15);
-                ok[q] = cell < n;
-                voff[q] = (int)((4 * g * ld + cell) * esz);
+                const int64_t cell0 = (ct0 + q) * 16 + (lane & 14);  // first cell of the lane pair
+                ok1[q] = cell0 < n;
+                ok2[q] = cell0 + 1 < n;
+                voff[q] = (int)(((4 * g + 2 * odd) * ld + cell0) * esz);
             }
 #pragma unroll
             for (int i = 0; i < RBW; i++) {
@@ -330,19 +360,21 @@ __global__ __launch_bounds__(64 * kRes4Waves) void split_resident_kernel(
                     static_cast<OT*>(out) + (int64_t)rb_row * ld, (short)0, (int)(16 * ld * esz), kBufWord3);
                 const v4f rsv = *reinterpret_cast<const v4f*>(rs + i * 16);  // this lane's 4 row scales
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const double rsc = (double)rsv[r];
+                for (int q = 0; q < TT; q++) {
+                    OT v[4];
 #pragma unroll
-                    for (int q = 0; q < TT; q++) {
-                        if (!ok[q] || (!kFull && rb_row + 4 * g + r >= rows)) continue;
-                        const double v =
-                            split_value(accm[i][q][r], accx[i][q][r]) * rsc * (double)cinv[q];
-                        if constexpr (sizeof(OT) == 4)
-                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)v), os,
-                                                                  voff[q], (int)(r * ld * esz), kNontemporal);
+                    for (int r = 0; r < 4; r++)
+                        v[r] = (OT)(split_value(accm[i][q][r], accx[i][q][r]) * (double)rsv[r] * (double)cinv[q]);
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const OT recv = swap_pair_lanes(odd ? v[h] : v[2 + h]);
+                        const OT first = odd ? recv : v[h];      // cell 2c
+                        const OT second = odd ? v[2 + h] : recv;  // cell 2c + 1
+                        if (!kFull && rb_row + 4 * g + 2 * odd + h >= rows) continue;
+                        if ((ct0 + q + 1) * 16 <= n)  // whole tile inside the output (uniform branch)
+                            store_cell_pair(first, second, true, true, os, voff[q], (int)(h * ld * esz));
                         else
-                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), os, voff[q],
-                                                                  (int)(r * ld * esz), kNontemporal);
+                            store_cell_pair(first, second, ok2[q], ok1[q], os, voff[q], (int)(h * ld * esz));
                     }
                 }
             }
@@ -488,8 +520,9 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
 #pragma unroll
                     for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
                 if constexpr (t + XD < STEPS) load_x(std::integral_constant<int, sl>{}, t + XD);
-                res_mfma_step<RBW, FMT, NT, kResTiles, NT>(lds + (int64_t)t * RBW * NT * 64 + lane, bx, nullptr, accm,
-                                                           accx);
+                V ahi[RBW];
+                res_mfma_step<RBW, FMT, NT, kResTiles, NT, false>(lds + (int64_t)t * RBW * NT * 64 + lane, nullptr, ahi,
+                                                                  bx, nullptr, accm, accx);
             });
             // This chunk's state (old values) and the output chunk through buffer
             // descriptors (uniform base, 32-bit offsets), kept before the next

@@ -56,6 +56,45 @@ __device__ __forceinline__ void order_after(V& x, const v4f& dep) {
     x = __builtin_bit_cast(V, u);
 }
 
+// Cache policy of the result stores: non-temporal (each result is written once
+// and not read again by the kernel that writes it).
+constexpr int kNontemporal = 2;
+
+// Value of the other lane of the pair (2i, 2i + 1): DPP quad permutation
+// [1, 0, 3, 2] (VALU, no LDS).
+__device__ __forceinline__ unsigned swap_pair_lanes(unsigned v) {
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float swap_pair_lanes(float v) {
+    return __builtin_bit_cast(float, swap_pair_lanes(__builtin_bit_cast(unsigned, v)));
+}
+__device__ __forceinline__ double swap_pair_lanes(double v) {
+    const v2u u = __builtin_bit_cast(v2u, v);
+    return __builtin_bit_cast(double, v2u{swap_pair_lanes(u[0]), swap_pair_lanes(u[1])});
+}
+
+// Stores two adjacent output cells (first at byte offset voff + soff of the
+// buffer, second after it) with one 16-byte (FP64) or 8-byte (FP32) store when
+// both are inside the output (both), the first alone when only it is (one).
+__device__ __forceinline__ void store_cell_pair(double first, double second, bool both, bool one,
+                                                __amdgpu_buffer_rsrc_t rsrc, int voff, int soff) {
+    if (both) {
+        const v2u a = __builtin_bit_cast(v2u, first), b = __builtin_bit_cast(v2u, second);
+        __builtin_amdgcn_raw_buffer_store_b128(v4u{a[0], a[1], b[0], b[1]}, rsrc, voff, soff, kNontemporal);
+    } else if (one) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, first), rsrc, voff, soff, kNontemporal);
+    }
+}
+__device__ __forceinline__ void store_cell_pair(float first, float second, bool both, bool one,
+                                                __amdgpu_buffer_rsrc_t rsrc, int voff, int soff) {
+    if (both)
+        __builtin_amdgcn_raw_buffer_store_b64(
+            v2u{__builtin_bit_cast(unsigned, first), __builtin_bit_cast(unsigned, second)}, rsrc, voff, soff,
+            kNontemporal);
+    else if (one)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, first), rsrc, voff, soff, kNontemporal);
+}
+
 template <int FMT>
 struct Fmt;
 template <>
