@@ -256,7 +256,8 @@ int rsh_dev_rel_error(const void* d_got, int32_t got_f32, int64_t ld_got, const 
 /* Achievable matrix-core rate of the device, TFLOP/s, from register-only loops
  * of independent MFMAs (8 waves per CU, no memory traffic): kind 0 =
  * v_mfma_f64_16x16x4f64 (the FP64 kernels' instruction), 1 =
- * v_mfma_f32_16x16x32_bf16 (the BF16 split forms'). Measures the peak that
+ * v_mfma_f32_16x16x32_bf16 (the BF16 split forms'). Three warm-up launches,
+ * then the fastest of five timed ones (~1 ms each). Measures the peak that
  * the roofline fractions divide by; no reference counterpart. */
 int rsh_mfma_peak(int32_t device, int32_t kind, double* tflops);
 int rsh_dev_synchronize(int32_t device);
@@ -307,12 +308,14 @@ typedef struct rsh_batch_s* rsh_batch;
 int rsh_batch_footprint(int32_t form, int64_t n_cells, int32_t m, int64_t* bytes);
 int rsh_batch_create(int32_t device, int32_t capacity, int32_t use_graph, rsh_batch* out);
 int rsh_batch_destroy(rsh_batch b);
-/* Kernel of the split-form slots added from now on: 0 = auto (the resident-
- * panel grouped kernel where its plan fits one XCD and is specialised: M = 200,
- * 400, 800 with M <= 800 and n_cells < 2^24; else the streaming-panel one),
- * 1 = resident panel only (rsh_batch_add fails where it does not apply),
- * 2 = streaming panel. Both give bit-identical states and residuals. No
- * reference counterpart (batching is an extension). */
+/* Kernel of the split-form slots added from now on: 0 = auto (the device's
+ * tuned choice for the slot's M, else the resident-panel grouped kernel with
+ * 8 waves where its plan fits one XCD and is specialised: M = 200, 400, 800
+ * with n_cells < 2^24; else the streaming-panel one), 1 = resident panel,
+ * 8 waves of 2 cell tiles (rsh_batch_add fails where it does not apply),
+ * 2 = streaming panel, 3 = resident panel, 4 waves of 4 cell tiles (fails as
+ * 1). All give bit-identical states and residuals. No reference counterpart
+ * (batching is an extension). */
 int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
 
 /* Autotuner of the grouped iteration launches. No reference counterpart (the
@@ -324,7 +327,8 @@ int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
  *                         (row blocks per pass) choose the MFMA tile (waves,
  *                         j-steps per LDS stage) among (4,2), (4,4), (8,2);
  *                         split forms choose, per M, the resident-panel
- *                         ({1,0}) or streaming-panel ({2,0}) grouped kernel.
+ *                         grouped kernel with 8 ({1,0}) or 4 waves ({3,0}) or
+ *                         the streaming-panel one ({2,0}).
  *                         Classes (FP64) and slots (split) created afterwards
  *                         in any batch on the device use the choice; every
  *                         candidate gives bit-identical states.
@@ -333,7 +337,7 @@ int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
  *   rsh_batch_set_f64_tile forces the FP64 tile of the classes this batch
  *                         creates from now on (NULL or {0,0}: tuned or default)
  *   rsh_batch_slot_config what a slot runs with: its class's FP64 tile, or
- *                         {1,0} / {2,0} for the split kernels */
+ *                         {1,0} / {3,0} / {2,0} for the split kernels */
 int rsh_batch_autotune(int32_t device, int32_t form, int32_t m, int64_t n_cells, int32_t iterations, rsh_tile* best,
                        float* best_ms);
 int rsh_batch_tuned(int32_t device, int32_t form, int32_t m, rsh_tile* choice);

@@ -70,6 +70,7 @@ struct Slot {
     F64Plan plan{};
     SplitPlan splan{};
     bool res = false;  // split form on the resident-panel grouped kernel
+    int res_waves = 8; // its waves per workgroup (8 or 4)
     PanelRef panel;    // the phase panel the device problem table points at
     int iterations = 0;
     int64_t bytes = 0;
@@ -102,7 +103,7 @@ struct rsh_batch_s {
     BatchProblem* d_probs = nullptr;
     unsigned long long* d_resid = nullptr;  // 2 * capacity
     unsigned* d_counters = nullptr;         // kMaxClasses * kXcdLists
-    int split_kernel = 0;                   // 0 auto, 1 resident panel, 2 streaming panel
+    int split_kernel = 0;                   // 0 auto, 1 resident panel, 2 streaming panel, 3 resident, 4 waves
     rsh_tile f64_tile{0, 0};                // FP64 grouped tile of new classes; {0, 0} = tuned or default
     std::map<std::pair<int, int>, ClassState> classes;  // by (form, row-block class)
     bool lists_dirty = true;
@@ -184,7 +185,9 @@ int slot_rb(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.rb_per_pass
 // Class key: (form, row blocks per pass) for the FP64 and streaming split
 // kernels; resident-panel split classes also separate by K-steps (one kernel
 // instance per plan).
-int slot_class(const Slot& s) { return s.res ? 1000 + 64 * s.splan.rb_per_pass + s.splan.steps : slot_rb(s); }
+int slot_class(const Slot& s) {
+    return s.res ? 1000 + (s.res_waves == 4 ? 100000 : 0) + 64 * s.splan.rb_per_pass + s.splan.steps : slot_rb(s);
+}
 int slot_passes(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.passes : s.splan.passes; }
 
 void destroy_graph(rsh_batch b) {
@@ -229,10 +232,10 @@ int rebuild(rsh_batch b) {
             } else if (s.res) {
                 cs.res = true;
                 cs.steps = s.splan.steps;
-                cs.tile = rsh_tile{8, 1};
+                cs.tile = rsh_tile{s.res_waves, 1};
                 cs.grid = device_cus(b->device) / kXcdLists * kXcdLists;  // one workgroup per CU (LDS-bound)
-                RSH_HIP_TRY(launch_grouped_iter_split_res(false, cs.form, rb, cs.steps, 0, nullptr, nullptr, nullptr,
-                                                          nullptr, 0, nullptr));
+                RSH_HIP_TRY(launch_grouped_iter_split_res(false, cs.form, rb, cs.steps, cs.tile.waves, 0, nullptr,
+                                                          nullptr, nullptr, nullptr, 0, nullptr));
                 RSH_HIP_TRY(hipMalloc(&cs.d_meta, 2 * kXcdLists * sizeof(int)));
             } else {
                 cs.tile = rsh_tile{grouped_iter_split_waves(), 1};
@@ -299,9 +302,9 @@ int enqueue_iteration(rsh_batch b, int parity) {
     for (auto& kv : b->classes) {
         ClassState& cs = kv.second;
         if (cs.res)
-            RSH_HIP_TRY(launch_grouped_iter_split_res(true, cs.form, cs.rb, cs.steps, cs.grid, b->d_probs, cs.d_items,
-                                                      cs.d_meta, b->d_counters + cs.counter_index, parity,
-                                                      b->stream));
+            RSH_HIP_TRY(launch_grouped_iter_split_res(true, cs.form, cs.rb, cs.steps, cs.tile.waves, cs.grid,
+                                                      b->d_probs, cs.d_items, cs.d_meta,
+                                                      b->d_counters + cs.counter_index, parity, b->stream));
         else if (cs.form == RSH_FORM_FP64)
             RSH_HIP_TRY(launch_grouped_iter_f64(cs.rb, cs.tile.waves, cs.tile.chunk_steps, cs.grid, b->d_probs,
                                                 cs.d_items, cs.d_n_items, b->d_counters + cs.counter_index, parity,
@@ -461,7 +464,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     int split_kernel = b->split_kernel;
     rsh_tile tuned{};
     if (form != RSH_FORM_FP64 && split_kernel == 0 && grouped_tuned(b->device, form, ds->m, &tuned))
-        split_kernel = tuned.waves == 2 ? 2 : 0;  // tuned streaming, else auto (resident where it applies)
+        split_kernel = tuned.waves == 2 || tuned.waves == 3 ? tuned.waves : 0;  // tuned, else auto (resident, 8 waves)
     if (form != RSH_FORM_FP64 && split_kernel != 2 && n_cells < (int64_t(1) << 24)) {
         // resident-panel grouped kernel where its plan fits and is specialised
         SplitPlan rplan{};
@@ -470,7 +473,8 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
             iter_split_res_supported(form, rplan.rb_per_pass, rplan.steps)) {
             s.splan = rplan;
             s.res = true;
-        } else if (split_kernel == 1) {
+            s.res_waves = split_kernel == 3 ? 4 : 8;
+        } else if (split_kernel == 1 || split_kernel == 3) {
             return fail(RSH_E_UNSUPPORTED, "resident-panel iteration kernel: no specialised plan for this M");
         }
     }
@@ -604,7 +608,7 @@ int rsh_batch_slot_config(rsh_batch b, int32_t slot, rsh_tile* config) {
         return fail(RSH_E_ARG, "invalid slot");
     const Slot& s = b->slots[slot];
     if (s.form != RSH_FORM_FP64) {
-        *config = rsh_tile{s.res ? 1 : 2, 0};
+        *config = rsh_tile{s.res ? (s.res_waves == 4 ? 3 : 1) : 2, 0};
         return RSH_OK;
     }
     RSH_HIP_TRY(hipSetDevice(b->device));
@@ -641,7 +645,7 @@ int rsh_batch_autotune(int32_t device, int32_t form, int32_t m, int64_t n_cells,
     if (form == RSH_FORM_FP64)
         cands = {{4, 2}, {4, 4}, {8, 2}};
     else
-        cands = {{1, 0}, {2, 0}};  // resident-panel, streaming-panel grouped kernel
+        cands = {{1, 0}, {3, 0}, {2, 0}};  // resident-panel (8 waves, 4 waves), streaming-panel grouped kernel
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e;
     if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
@@ -698,7 +702,8 @@ int rsh_batch_autotune(int32_t device, int32_t form, int32_t m, int64_t n_cells,
 }
 
 int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind) {
-    if (!b || kind < 0 || kind > 2) return fail(RSH_E_ARG, "split kernel: 0 (auto), 1 (resident panel), 2 (streaming)");
+    if (!b || kind < 0 || kind > 3)
+        return fail(RSH_E_ARG, "split kernel: 0 (auto), 1 (resident panel), 2 (streaming), 3 (resident panel, 4 waves)");
     b->split_kernel = kind;
     return RSH_OK;
 }

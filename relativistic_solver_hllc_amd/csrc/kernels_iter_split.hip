@@ -5,7 +5,8 @@
 // Epilogue, per output (k, c): S in FP64 from the main and cross FP32
 // sums and the exact scales; I_new = (1 - a) B + a S in FP64; the old
 // value is the state the MFMA consumed (parts summed, rescaled exactly); the
-// residual |I_old - I_new| / I_new is FP64; I_new is re-split into the packed
+// residual |I_old - I_new| / I_new is FP64 (its maximum kept as a ratio,
+// RelMax); I_new is re-split into the packed
 // layout of the other ping-pong buffer.
 //
 // Packed position of (row k, cell c): cell tile c / 16, K-step k / 32, vector
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
         split_accumulate<RB, W, FMT, NT, 1, 1>(st_in, P.n, P.steps,
                                                static_cast<const v4u*>(P.spanel) + (int64_t)item.pass * P.steps * UNITS,
                                                ct, 0, lds, accm, accx);
-        double res = 0.0;
+        RelMax rm;
         if (cell < P.n) {
             const double cinv = (double)P.cell_inv[cell];
             const double cscale = 1.0 / cinv;  // power of two: exact
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
 #pragma unroll
                     for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
                     const double old = (double)of * cinv;
-                    res = fmax(res, fabs(old - nv) / nv);
+                    rm.add(old - nv, nv, true);
                     T q[NT];
                     split_parts<FMT, NT>((float)(nv * cscale), q);
 #pragma unroll
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 }
             }
         }
-        res = wave_max_split(res);
+        const double res = wave_max_split(rm.value());
         if (lane == 0 && res > 0.0)
             atomicMax(P.resid + parity, (unsigned long long)__double_as_longlong(res));
     }

@@ -56,6 +56,9 @@ constexpr int kBufWord3 = 0x00020000;
 template <int NT>
 constexpr int res_xd() { return NT == 2 ? 3 : 2; }
 constexpr int kResLoXd = 1;
+#ifndef GXD
+#define GXD 0
+#endif
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -415,13 +418,15 @@ hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* s
 // the two accumulators and the exact scales, I_new = (1 - a) B + a S, the
 // residual against the state the MFMAs consumed, I_new re-split into the other
 // ping-pong buffer. Same products, same order: bit-identical to it.
-template <int RBW, int FMT, int NT, int STEPS>
-__global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
+template <int RBW, int FMT, int NT, int STEPS, int W>
+__global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, const int* __restrict__ meta,
     unsigned* __restrict__ counters, int parity) {
     typedef typename Fmt<FMT>::V V;
     typedef typename Fmt<FMT>::T T;
-    constexpr int XD = NT == 2 ? 3 : 2;  // shallower than the plain kernel: the epilogue needs the registers
+    constexpr int TT = kResChunkTiles / W;  // 16-cell tiles per wave
+    static_assert(TT * W == kResChunkTiles, "waves must divide the 16 tiles of a chunk");
+    constexpr int XD = (GXD > 0 && STEPS >= 25) ? GXD : (NT == 2 ? 3 : 2);  // shallower than the plain kernel: the epilogue needs the registers
     constexpr int steps = STEPS;
     constexpr int64_t units = (int64_t)steps * RBW * NT * 64;
     extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then 16 RBW row scales
@@ -446,8 +451,8 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
         const int rg = item.pass;
         const int row0 = rg * RBW * 16;
         const v4u* __restrict__ src = static_cast<const v4u*>(P.spanel) + (int64_t)rg * units;
-        for (int64_t u = threadIdx.x; u < units; u += 64 * kResWaves) lds[u] = src[u];
-        for (int r = threadIdx.x; r < RBW * 16; r += 64 * kResWaves)
+        for (int64_t u = threadIdx.x; u < units; u += 64 * W) lds[u] = src[u];
+        for (int r = threadIdx.x; r < RBW * 16; r += 64 * W)
             rscale[r] = row0 + r < P.m ? P.row_inv[row0 + r] : 0.0f;
         __syncthreads();
 
@@ -460,7 +465,7 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
         const int64_t c1 = c0 + kIterResSub < chunks ? c0 + kIterResSub : chunks;
 
         __amdgpu_buffer_rsrc_t xs;
-        int xoff[kResTiles];
+        int xoff[TT];
         int64_t ntile_of_bound = 0;
         auto bind_chunk = [&](int64_t chunk) {
             const int64_t ct_chunk = chunk * kResChunkTiles;
@@ -469,16 +474,16 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
             xs = __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(st_in + ct_chunk * tile_units), (short)0,
                                                    ntile * tile_bytes, kBufWord3);
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++) {
-                const int tile = wave * kResTiles + q;
+            for (int q = 0; q < TT; q++) {
+                const int tile = wave * TT + q;
                 xoff[q] = (tile < ntile ? tile : ntile - 1) * tile_bytes + lane * 16;
             }
         };
-        v4u xr[XD][kResTiles][NT];
+        v4u xr[XD][TT][NT];
         auto load_x = [&](auto slot_tag, int t) {
             constexpr int sl = decltype(slot_tag)::value;
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++)
+            for (int q = 0; q < TT; q++)
 #pragma unroll
                 for (int p = 0; p < NT; p++)
                     xr[sl][q][p] = __builtin_bit_cast(
@@ -489,24 +494,24 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
                 if (u.value < steps) load_x(u, u.value);
             });
         };
-        double res = 0.0;
+        RelMax rm;
         if (c0 < c1) {
             bind_chunk(c0);
             prefetch_head();
         }
         for (int64_t chunk = c0; chunk < c1; chunk++) {
-            const int64_t ct0 = chunk * kResChunkTiles + wave * kResTiles;
-            float cinv[kResTiles];
+            const int64_t ct0 = chunk * kResChunkTiles + wave * TT;
+            float cinv[TT];
 #pragma unroll
-            for (int q = 0; q < kResTiles; q++) {
+            for (int q = 0; q < TT; q++) {
                 const int64_t cell = (ct0 + q) * 16 + cl;
                 cinv[q] = P.cell_inv[cell < n ? cell : n - 1];
             }
-            v4f accm[RBW][kResTiles], accx[RBW][kResTiles];
+            v4f accm[RBW][TT], accx[RBW][TT];
 #pragma unroll
             for (int i = 0; i < RBW; i++)
 #pragma unroll
-                for (int q = 0; q < kResTiles; q++) {
+                for (int q = 0; q < TT; q++) {
                     accm[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
                     accx[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
                 }
@@ -514,14 +519,14 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
                 constexpr int t = decltype(tt)::value;
                 constexpr int sl = t % XD;
                 __builtin_amdgcn_sched_barrier(0);
-                V bx[kResTiles][NT];
+                V bx[TT][NT];
 #pragma unroll
-                for (int q = 0; q < kResTiles; q++)
+                for (int q = 0; q < TT; q++)
 #pragma unroll
                     for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
                 if constexpr (t + XD < STEPS) load_x(std::integral_constant<int, sl>{}, t + XD);
                 V ahi[RBW];
-                res_mfma_step<RBW, FMT, NT, kResTiles, NT, false>(lds + (int64_t)t * RBW * NT * 64 + lane, nullptr, ahi,
+                res_mfma_step<RBW, FMT, NT, TT, NT, false>(lds + (int64_t)t * RBW * NT * 64 + lane, nullptr, ahi,
                                                                   bx, nullptr, accm, accx);
             });
             // This chunk's state (old values) and the output chunk through buffer
@@ -541,67 +546,95 @@ __global__ __launch_bounds__(64 * kResWaves) void grouped_iter_split_res_kernel(
             asm volatile("v_mov_b32 %0, 0" : "=v"(zoff));
             const float* __restrict__ rs = rscale + 4 * g + zoff;
             const double oma = P.one_minus_albedo, alb = P.albedo;  // in registers across the stores
-#pragma unroll
-            for (int q = 0; q < kResTiles; q++) {
-                const int tl = wave * kResTiles + q;  // tile within the chunk
+            // Branch-free epilogue: every load is issued with an address that is
+            // in range (cell clamped to the last cell, a padding row block to the
+            // last real one), results of invalid positions are dropped by giving
+            // their stores an offset past the buffer's range (the hardware
+            // discards such stores) and by leaving them out of the residual. So
+            // the whole epilogue is one basic block, and the scheduler overlaps the
+            // loads of later (tile, row block) pairs with the arithmetic of earlier
+            // ones instead of waiting for one memory round trip per pair.
+            const int rb_last = (P.m + 15) / 16 - 1;  // last row block with rows
+            constexpr int kDropped = 0x7FFF0000;      // store offset past any buffer's range
+            // (tile q, row block i) pairs in order p = q * RBW + i; the loads of
+            // pair p + LA are issued before the stores of pair p (the compiler
+            // may not move a load above a store to another buffer: it cannot tell
+            // that they do not alias). LA: as far ahead as the registers allow.
+            constexpr int NPAIR = TT * RBW;
+            constexpr int LA = RBW <= 3 ? 3 : (RBW == 4 ? 2 : 1);
+            double bv[LA + 1][4];
+            v2u oldraw[LA + 1][NT];
+            auto pair_rb = [&](int i, bool& rb_ok) {
+                const int rbg = rg * RBW + i;
+                rb_ok = rbg <= rb_last;  // a padding row block has no rows and no state
+                return rb_ok ? rbg : rb_last;
+            };
+            auto pair_voff = [&](int q, int rbe) {
+                const int vlane = (2 * (rbe & 1) + (g >> 1)) * 16 + cl;
+                return ((wave * TT + q) * (int)tile_units + vlane) * 16 + 8 * (g & 1);
+            };
+            auto load_pair = [&](auto pp) {
+                constexpr int pi = decltype(pp)::value, q = pi / RBW, i = pi % RBW, buf = pi % (LA + 1);
+                bool rb_ok;
+                const int rbe = pair_rb(i, rb_ok);
                 const int64_t cell = (ct0 + q) * 16 + cl;
-                if (tl >= ntile_cur || cell >= n) continue;
+                const int cbyte = (int)((cell < n ? cell : n - 1) * 8);
+                const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<double*>(P.B) + (int64_t)rbe * 16 * n, (short)0, (int)(16 * n * 8), kBufWord3);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    bv[buf][r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                                bsrc, (int)(4 * g * n * 8) + cbyte, (int)(r * n * 8), 0));
+                const int voff = pair_voff(q, rbe);
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    oldraw[buf][t] = __builtin_bit_cast(
+                        v2u, __builtin_amdgcn_raw_buffer_load_b64(xin, voff, ((rbe >> 1) * NT + t) * 1024, 0));
+            };
+            static_for<0, (LA < NPAIR ? LA : NPAIR)>(load_pair);
+            static_for<0, NPAIR>([&](auto pp) {
+                constexpr int pi = decltype(pp)::value, q = pi / RBW, i = pi % RBW, buf = pi % (LA + 1);
+                if constexpr (pi + LA < NPAIR) load_pair(std::integral_constant<int, pi + LA>{});
+                const int tl = wave * TT + q;  // tile within the chunk
+                const int64_t cell = (ct0 + q) * 16 + cl;
+                bool rb_ok;
+                const int rbe = pair_rb(i, rb_ok);
+                // B rows exist up to the row block's end (the source array holds
+                // whole row blocks, padding rows zero). A padding row (k >= M)
+                // has zero panel rows, zero B and zero old state: nv = 0, parts
+                // 0, residual term 0 / 0 = NaN, which fmax drops.
+                const bool ok = tl < ntile_cur && cell < n && rb_ok;
                 const double ci = (double)cinv[q];
                 const double cscale = 1.0 / ci;  // power of two: exact
-                const int cbyte = (int)(cell * 8);
+                T newp[NT][4];
 #pragma unroll
-                for (int i = 0; i < RBW; i++) {
-                    const int rbg = rg * RBW + i;
-                    if (rbg * 16 >= P.m) continue;  // padding row block: no rows, no state
-                    const int s = rbg >> 1;
-                    const int vlane = (2 * (rbg & 1) + (g >> 1)) * 16 + cl;
-                    const int voff = (tl * (int)tile_units + vlane) * 16 + 8 * (g & 1);
-                    const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
-                        const_cast<double*>(P.B) + (int64_t)rbg * 16 * n, (short)0, (int)(16 * n * 8), kBufWord3);
-                    // B rows exist up to the row block's end (the source array
-                    // holds whole row blocks, padding rows zero), so all loads of
-                    // the row block are issued together, with no per-row branch.
-                    // A padding row (k >= M) has zero panel rows, zero B and zero
-                    // old state: nv = 0, parts 0, residual term 0 / 0 = NaN,
-                    // which fmax drops.
-                    double bv[4];
+                for (int r = 0; r < 4; r++) {
+                    const double S = split_value(accm[i][q][r], accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
+                    const double nv = oma * bv[buf][r] + alb * S;
+                    float of = 0.0f;
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        bv[r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
-                                                               bsrc, (int)(4 * g * n * 8) + cbyte, (int)(r * n * 8), 0));
-                    T oldp[NT][4], newp[NT][4];
-#pragma unroll
-                    for (int t = 0; t < NT; t++) {
-                        const v2u raw = __builtin_bit_cast(
-                            v2u, __builtin_amdgcn_raw_buffer_load_b64(xin, voff, (s * NT + t) * 1024, 0));
+                    for (int t = NT - 1; t >= 0; t--) {
                         T tmp[4];
-                        __builtin_memcpy(tmp, &raw, 8);
-#pragma unroll
-                        for (int r = 0; r < 4; r++) oldp[t][r] = tmp[r];
+                        __builtin_memcpy(tmp, &oldraw[buf][t], 8);
+                        of += (float)tmp[r];
                     }
+                    const double old = (double)of * ci;
+                    rm.add(old - nv, nv, ok);
+                    T qp[NT];
+                    split_parts<FMT, NT>((float)(nv * cscale), qp);
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const double S = split_value(accm[i][q][r], accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
-                        const double nv = oma * bv[r] + alb * S;
-                        float of = 0.0f;
-#pragma unroll
-                        for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
-                        const double old = (double)of * ci;
-                        res = fmax(res, fabs(old - nv) / nv);
-                        T qp[NT];
-                        split_parts<FMT, NT>((float)(nv * cscale), qp);
-#pragma unroll
-                        for (int t = 0; t < NT; t++) newp[t][r] = qp[t];
-                    }
-#pragma unroll
-                    for (int t = 0; t < NT; t++) {
-                        v2u raw;
-                        __builtin_memcpy(&raw, newp[t], 8);
-                        __builtin_amdgcn_raw_buffer_store_b64(raw, xout, voff, (s * NT + t) * 1024, 0);
-                    }
+                    for (int t = 0; t < NT; t++) newp[t][r] = qp[t];
                 }
-            }
+                const int soff = ok ? pair_voff(q, rbe) : kDropped;
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    v2u raw;
+                    __builtin_memcpy(&raw, newp[t], 8);
+                    __builtin_amdgcn_raw_buffer_store_b64(raw, xout, soff, ((rbe >> 1) * NT + t) * 1024, 0);
+                }
+            });
         }
+        double res = rm.value();
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) res = fmax(res, __shfl_xor(res, off, 64));
         if (lane == 0 && res > 0.0)
@@ -699,23 +732,25 @@ size_t iter_split_res_lds(int form, int rb, int steps) {
 }
 
 template <int RBW, int FMT, int NT, int STEPS>
-static hipError_t iter_res_one(bool launch, int grid, size_t lds, hipStream_t st, const BatchProblem* probs,
+static hipError_t iter_res_one(bool launch, int waves, int grid, size_t lds, hipStream_t st, const BatchProblem* probs,
                                const BatchItem* items, const int* meta, unsigned* counters, int parity) {
-    auto kern = grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS>;
+    auto kern = waves == 4 ? grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, 4>
+                           : grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, kResWaves>;
     if (!launch)
         return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kResWaves), lds, st, probs, items, meta, counters, parity);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, st, probs, items, meta, counters, parity);
     return hipGetLastError();
 }
 
 // launch = false: only sets the kernel's dynamic-LDS limit (call once, outside graph capture).
-hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int grid,
+hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int waves, int grid,
                                          const BatchProblem* d_probs, const BatchItem* d_items, const int* d_meta,
                                          unsigned* d_counters, int parity, hipStream_t st) {
     const int fmt = split_fmt(form), nt = split_terms(form);
+    if (waves != 4 && waves != kResWaves) return hipErrorInvalidValue;
     const size_t lds = iter_split_res_lds(form, rb, steps);
-#define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
+#define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, waves, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
     RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(3, 0, 2, 25)
     RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(3, 1, 2, 25)
     RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(2, 0, 3, 25)

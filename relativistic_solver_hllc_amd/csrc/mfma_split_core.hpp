@@ -60,6 +60,23 @@ __device__ __forceinline__ void order_after(V& x, const v4f& dep) {
 // and not read again by the kernel that writes it).
 constexpr int kNontemporal = 2;
 
+// Running maximum of |d| / v (the source iteration's relative change) over the
+// results one lane owns, kept as the pair (num, den) of the largest ratio so
+// far: a term wins when |d| den > num v (two products instead of an FP64
+// division per result); value() divides once. As with fmax over the quotients,
+// a NaN term, a padding row's 0 / 0 and a negative v never win, and v = 0 <
+// |d| gives +inf.
+struct RelMax {
+    double num = 0.0, den = 1.0;
+    __device__ __forceinline__ void add(double d, double v, bool ok) {
+        const double a = fabs(d);
+        const bool win = ok && v >= 0.0 && a * den > num * v;
+        num = win ? a : num;
+        den = win ? v : den;
+    }
+    __device__ __forceinline__ double value() const { return num / den; }
+};
+
 // Value of the other lane of the pair (2i, 2i + 1): DPP quad permutation
 // [1, 0, 3, 2] (VALU, no LDS).
 __device__ __forceinline__ unsigned swap_pair_lanes(unsigned v) {

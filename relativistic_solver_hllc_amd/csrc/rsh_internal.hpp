@@ -194,9 +194,11 @@ int grouped_iter_split_resident(int form, int rb, int device);
 // Grouped iteration on split state, resident-panel form (kernels_split_resident.hip):
 // items (problem, row group, chunk of iter_split_res_chunk_cells() cells) in 8
 // per-XCD lists; meta = [8 list begins, 8 list counts]; counters = 8 per class.
+// waves: 8 (two tiles of 16 cells per wave, two waves per SIMD) or 4 (four
+// tiles per wave, one wave per SIMD); same items, panel and results.
 bool iter_split_res_supported(int form, int rb, int steps);
 size_t iter_split_res_lds(int form, int rb, int steps);
-hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int grid,
+hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int steps, int waves, int grid,
                                          const BatchProblem* d_probs, const BatchItem* d_items, const int* d_meta,
                                          unsigned* d_counters, int parity, hipStream_t st);
 int iter_split_res_chunk_cells();

@@ -257,7 +257,8 @@ def tuned_tile_f64(m: int, device: int = 0) -> Tuple[int, int]:
 
 def batch_autotune(form: str, m: int, n_cells: int = 100000, iterations: int = 6, device: int = 0):
     """Times the grouped-launch candidates of the batch class of (form, m) and records the fastest
-    for the device: FP64 -> ("tile", (waves, j-steps)); split forms -> ("kernel", "resident" | "streaming").
+    for the device: FP64 -> ("tile", (waves, j-steps)); split forms -> ("kernel", "resident" | "resident4" |
+    "streaming").
     Returns (choice, ms per iteration)."""
     t, ms = N.Tile(), C.c_float()
     N.call("rsh_batch_autotune", device, FORMS[form], m, int(n_cells), iterations, C.byref(t), C.byref(ms))
@@ -273,7 +274,7 @@ def batch_tuned(form: str, m: int, device: int = 0):
 def _grouped_choice(form: str, t):
     if form == "fp64":
         return ("tile", (t.waves, t.chunk_steps))
-    return ("kernel", {0: "auto", 1: "resident", 2: "streaming"}[t.waves])
+    return ("kernel", {0: "auto", 1: "resident", 2: "streaming", 3: "resident4"}[t.waves])
 
 
 class Batch:
@@ -304,8 +305,10 @@ class Batch:
         return slot.value
 
     def set_split_kernel(self, kind: str) -> None:
-        """Kernel for split-form slots added afterwards: "auto", "resident" or "streaming"."""
-        N.call("rsh_batch_set_split_kernel", self.handle, {"auto": 0, "resident": 1, "streaming": 2}[kind])
+        """Kernel for split-form slots added afterwards: "auto", "resident" (resident panel, 8 waves of 2 cell
+        tiles), "resident4" (resident panel, 4 waves of 4 cell tiles) or "streaming"."""
+        N.call("rsh_batch_set_split_kernel", self.handle,
+               {"auto": 0, "resident": 1, "streaming": 2, "resident4": 3}[kind])
 
     def set_f64_tile(self, tile: Optional[Tuple[int, int]]) -> None:
         """FP64 grouped tile (waves, j-steps) of the classes created from now on; None = tuned or default."""

@@ -281,29 +281,37 @@ def test_arena_reuse_and_exhaustion(gpu):
 
 @pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
 def test_split_resident_kernel_equals_streaming(gpu, form):
-    """The resident-panel grouped kernel (M = 200, 400, 800) and the streaming-panel
-    one compute the same products in the same order: after several graph-replayed
-    iterations of an uneven mix (ragged cell counts, one M without a resident
-    plan), states and residuals are bit-identical."""
+    """The resident-panel grouped kernel (M = 200, 400, 800; 8 waves of 2 cell tiles
+    or 4 waves of 4) and the streaming-panel one compute the same products in the
+    same order: after several graph-replayed iterations of an uneven mix (ragged
+    cell counts, one M without a resident plan), states and residuals are
+    bit-identical."""
     cases = [(400, 2000, "hg", 0.6), (200, 1025, "thomson", 0.0), (800, 333, "hg", 0.9), (161, 257, "hg", 0.3),
              (400, 17, "hg", 0.8), (200, 5000, "hg", 0.3)]
     probs = [make(m, n, ph, g, seed=90 + i) for i, (m, n, ph, g) in enumerate(cases)]
     out = {}
-    for kind in ("streaming", "resident", "auto"):
+    for kind in ("streaming", "resident", "resident4", "auto"):
         with rs.Batch(8) as b:
-            b.set_split_kernel(kind if kind != "resident" else "auto")
-            slots = [add_form(b, p, form) for p in probs]
+            slots = []
+            for p in probs:
+                # the resident kinds where a resident plan exists (M = 161 has none), else auto
+                b.set_split_kernel(kind if p["m"] != 161 or kind == "streaming" else "auto")
+                slots.append(add_form(b, p, form))
+            for s, p in zip(slots, probs):
+                if kind in ("resident", "resident4", "streaming") and p["m"] != 161:
+                    assert b.slot_config(s, form) == ("kernel", kind)
             b.run(5)
             r, it = b.residuals()
             out[kind] = ([state(b, s, p) for s, p in zip(slots, probs)], [r[s] for s in slots])
-    for k in ("resident", "auto"):
+    for k in ("resident", "resident4", "auto"):
         for a, c in zip(out["streaming"][0], out[k][0]):
             assert np.array_equal(a, c)
         assert out["streaming"][1] == out[k][1]
-    with rs.Batch(2) as b:
-        b.set_split_kernel("resident")
-        with pytest.raises(rs.N.RshError):
-            add_form(b, probs[3], form)        # M = 161: no specialised resident plan
+    for kind in ("resident", "resident4"):
+        with rs.Batch(2) as b:
+            b.set_split_kernel(kind)
+            with pytest.raises(rs.N.RshError):
+                add_form(b, probs[3], form)    # M = 161: no specialised resident plan
     for p in probs:
         p["ds"].close()
 
@@ -381,7 +389,7 @@ def test_batch_autotune_choice_is_used_and_every_tile_agrees(gpu):
         assert np.array_equal(states[tile], states[None])
     for form in ("bf16x3", "fp16x2"):
         kchoice, kms = rs.batch_autotune(form, 400, n_cells=20000, iterations=4)
-        assert kchoice[0] == "kernel" and kchoice[1] in ("resident", "streaming") and kms > 0
+        assert kchoice[0] == "kernel" and kchoice[1] in ("resident", "resident4", "streaming") and kms > 0
         with rs.Batch(2) as b:
             s = add_form(b, p, form)
             assert b.slot_config(s, form) == kchoice
