@@ -217,6 +217,90 @@ __global__ __launch_bounds__(256) void pack_fused_kernel(const double* __restric
     }
 }
 
+// Wide form of the fused pack for M <= 32 kPackWideMaxSteps: a workgroup of
+// 16 CPB threads packs CPB / 16 tiles (CPB cells) and keeps the face averages
+// in registers instead of LDS. Thread slot u = t + 16 CPB i stands for
+// (K-step s = u / (4 CPB), row group g = (u / CPB) % 4, cell u % CPB), i.e.
+// the 8 directions 32 s + 8 g + e of one cell that one state vector lane
+// holds: the thread reads exactly the values it packs, so no transposition
+// through LDS is needed, and a load instruction reads CPB consecutive cells of
+// a direction row (32 CPB contiguous bytes; 512 in the 16-cell LDS form).
+// Same arithmetic, same bits as pack_fused_kernel.
+constexpr int kPackWideMaxSteps = 16;  // 4 slots per thread: 8 x 4 doubles in registers
+constexpr int kPackCells = 64;  // cells per workgroup (32 measured the same, 16 2 % slower)
+template <int FMT, int NT, int F, int MAXS, int CPB>
+__global__ __launch_bounds__(16 * CPB) void pack_wide_kernel(const double* __restrict__ illum,
+                                                             const double* __restrict__ extra, int64_t n, int m,
+                                                             int steps, float* __restrict__ inv_scale,
+                                                             v4u* __restrict__ state) {
+    typedef typename Fmt<FMT>::T T;
+    constexpr int THREADS = 16 * CPB;
+    __shared__ double cmax[16][CPB];
+    const int t = threadIdx.x;
+    const int cc = t % CPB;  // the thread's cell in the group (every slot of a thread has the same cell)
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t cell = (int64_t)blockIdx.x * CPB + cc;
+    const bool cvalid = cell < n;
+    const int64_t cell_c = cvalid ? cell : n - 1;  // invalid cells re-read a valid one, results unused
+    const int nslots = steps * 4 * CPB;
+    double avg[MAXS][8];
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXS; i++) {
+        const int u = t + THREADS * i;
+        const int j0 = 32 * (u / (4 * CPB)) + 8 * ((u / CPB) % 4);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int j = j0 + e;
+            double a = 0.0;
+            if (u < nslots && j < m) {
+                const double* p = illum + ((int64_t)j * n + cell_c) * F;
+                if constexpr (F == 4) {
+                    const double2 lo = *reinterpret_cast<const double2*>(p);
+                    const double2 hi = *reinterpret_cast<const double2*>(p + 2);
+                    a = (((lo.x + lo.y) + hi.x) + hi.y) * 0.25;
+                    mx = fmax(mx, fmax(fmax(fabs(lo.x), fabs(lo.y)), fmax(fabs(hi.x), fabs(hi.y))));
+                } else {
+                    a = p[0];
+                    mx = fmax(mx, fabs(p[0]));
+                    if (extra) mx = fmax(mx, fabs(extra[(int64_t)j * n + cell_c]));
+                }
+            }
+            avg[i][e] = cvalid ? a : 0.0;
+        }
+    }
+    // the cell's maximum over the 16 threads that read it
+    cmax[t / CPB][cc] = mx;
+    __syncthreads();
+    double cm = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) cm = fmax(cm, cmax[w][cc]);
+    const int e = cm > 0.0 ? (kTargetExp - 1 - ilogb(cm)) : 0;
+    const float inv = ldexpf(1.0f, -e);
+    if (t < CPB && cvalid) inv_scale[cell] = inv;
+    const double scale = cvalid ? 1.0 / (double)inv : 0.0;
+    const int64_t tile = (int64_t)blockIdx.x * (CPB / 16) + cc / 16;
+    if (tile >= tiles) return;  // no barrier below
+    v4u* out = state + tile * steps * NT * 64;
+#pragma unroll
+    for (int i = 0; i < MAXS; i++) {
+        const int u = t + THREADS * i;
+        if (u >= nslots) break;
+        const int sk = u / (4 * CPB), lane = ((u / CPB) % 4) * 16 + cc % 16;
+        T buf[NT][8];
+#pragma unroll
+        for (int e8 = 0; e8 < 8; e8++) {
+            const float x = (float)(avg[i][e8] * scale);
+            T parts[NT];
+            split_parts<FMT, NT>(x, parts);
+#pragma unroll
+            for (int q = 0; q < NT; q++) buf[q][e8] = parts[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) out[(sk * NT + q) * 64 + lane] = __builtin_bit_cast(v4u, buf[q]);
+    }
+}
+
 // ------------------------------------------------------------------ phase panel
 // One block per output row: row scale from the row maximum, then the fragments.
 template <int FMT, int NT>
@@ -378,10 +462,23 @@ hipError_t launch_pack_fused(int form, const double* d_illum, const double* d_ex
     if (!pack_fused_supported(m) || (faces != 1 && faces != 4) || (faces == 4 && d_extra))
         return hipErrorInvalidValue;
     const int steps = (m + 31) / 32;
-    const size_t lds = (size_t)steps * 32 * 16 * sizeof(double);
-    const dim3 grid((unsigned)((n + 15) / 16)), block(256);
     v4u* out = static_cast<v4u*>(d_state);
     const int fmt = split_fmt(form), nt = split_terms(form);
+    if (steps <= kPackWideMaxSteps) {
+        const dim3 wgrid((unsigned)((n + kPackCells - 1) / kPackCells)), wblock(16 * kPackCells);
+#define RSH_PW(FMT, NT, F)                                                                                        \
+    {                                                                                                             \
+        hipLaunchKernelGGL((pack_wide_kernel<FMT, NT, F, kPackWideMaxSteps / 4, kPackCells>), wgrid, wblock, 0, st, \
+                           d_illum, d_extra, n, m, steps, d_inv, out);                                            \
+        return hipGetLastError();                                                                                 \
+    }
+        if (fmt == 0 && nt == 2) { if (faces == 4) RSH_PW(0, 2, 4) else RSH_PW(0, 2, 1) }
+        if (fmt == 1 && nt == 2) { if (faces == 4) RSH_PW(1, 2, 4) else RSH_PW(1, 2, 1) }
+        if (faces == 4) RSH_PW(0, 3, 4) else RSH_PW(0, 3, 1)
+#undef RSH_PW
+    }
+    const size_t lds = (size_t)steps * 32 * 16 * sizeof(double);
+    const dim3 grid((unsigned)((n + 15) / 16)), block(256);
 #define RSH_PF(FMT, NT, F)                                                                                       \
     {                                                                                                            \
         auto k = pack_fused_kernel<FMT, NT, F>;                                                                  \
