@@ -472,7 +472,7 @@ def batch_section(D, rs, iters=20):
         tuned[form] = {str(mm): rs.batch_autotune(form, mm)[0][1] for mm in sorted({p["m"] for p in probs})}
     res["autotune"] = {"choices": tuned, "seconds": time.perf_counter() - t0,
                        "candidates": "FP64: MFMA tiles (waves, j-steps per LDS stage) (4,2), (4,4), (8,2); "
-                                     "split: resident-panel vs streaming-panel grouped kernel"}
+                                     "split: resident-panel grouped kernel with 8 or 4 waves vs streaming-panel grouped kernel"}
     log(f"[bench] batch autotune: {tuned}")
     states = {}
     for form in ("fp64", "bf16x2", "fp16x2", "bf16x3"):
