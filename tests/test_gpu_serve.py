@@ -1,7 +1,8 @@
 """GPU serving loop (rsh_server_*): continuous batching of source-iteration requests.
 
 * a request's result is the same alone and in a mix (bit-identical state and
-  iteration count: batching changes neither the kernels nor the summation order);
+  iteration count, FP64 and split forms: batching changes neither the kernels
+  nor the summation order);
 * every request converges below its FP64 residual tolerance and matches the
   oracle's iteration of the same seeded problem (state within 1e-12 for FP64);
 * admission respects the HBM budget; cancellation frees the slot;
@@ -36,26 +37,32 @@ def drain(srv, limit=500):
     raise AssertionError("server did not drain")
 
 
-def test_alone_equals_in_mix(gpu):
-    target = rs.request(3000, 400, "hg", 0.6, tolerance=1e-8, seed=5)
-    others = [rs.request(1000 + 700 * i, m, ph, g, tolerance=1e-8, seed=100 + i)
+@pytest.mark.parametrize("form,tol,n_target,scale", [("fp64", 1e-8, 3000, 1), ("bf16x3", 1e-6, 3000, 1),
+                                                     ("fp16x2", 1e-6, 250000, 40)])
+def test_alone_equals_in_mix(gpu, form, tol, n_target, scale):
+    """A request's state and iteration count are the same alone and in a mix that
+    arrives around it (FP64 and split forms; the last case at 250,000 cells with
+    a mix of 40,000-220,000-cell requests)."""
+    target = rs.request(n_target, 400, "hg", 0.6, tolerance=tol, form=form, seed=5)
+    others = [rs.request((1000 + 700 * i) * scale, m, ph, g, tolerance=tol, form=form, seed=100 + i)
               for i, (m, ph, g) in enumerate([(200, "thomson", 0.0), (800, "hg", 0.9), (400, "hg", 0.3),
                                               (200, "hg", 0.6)])]
     with rs.Server(64 * GB, 8, iters_per_step=2, keep_results=True) as srv:
         rid = srv.submit(target)
         drain(srv)
-        alone, info_alone = result(srv, rid, 400, 3000), srv.poll(rid)
+        alone, info_alone = result(srv, rid, 400, n_target), srv.poll(rid)
     with rs.Server(64 * GB, 8, iters_per_step=2, keep_results=True) as srv:
         ids = [srv.submit(others[0]), srv.submit(others[1])]
         srv.step()                      # the target arrives while others are on the GPU
         rid = srv.submit(target)
         ids += [srv.submit(o) for o in others[2:]]
         drain(srv)
-        mixed, info_mix = result(srv, rid, 400, 3000), srv.poll(rid)
+        mixed, info_mix = result(srv, rid, 400, n_target), srv.poll(rid)
         for i in ids:
             assert srv.poll(i)["status"] == "converged"
     assert info_alone["status"] == info_mix["status"] == "converged"
     assert info_alone["iterations"] == info_mix["iterations"]
+    assert info_alone["residual"] < tol
     assert np.array_equal(alone, mixed)
 
 
