@@ -467,9 +467,10 @@ def batch_section(D, rs, iters=20):
     # Grouped-launch configurations chosen by the autotuner on this device (untimed):
     # FP64 per row-block class (M = 200, 400, 800 share one), split forms per M.
     t0 = time.perf_counter()
-    tuned = {"fp64": {"400": rs.batch_autotune("fp64", 400)[0][1]}}
+    tuned = {"fp64": {"400": rs.batch_autotune("fp64", 400, n_cells=300000, iterations=4)[0][1]}}
     for form in ("bf16x2", "fp16x2", "bf16x3"):
-        tuned[form] = {str(mm): rs.batch_autotune(form, mm)[0][1] for mm in sorted({p["m"] for p in probs})}
+        tuned[form] = {str(mm): rs.batch_autotune(form, mm, n_cells=300000, iterations=4)[0][1]
+                       for mm in sorted({p["m"] for p in probs})}
     res["autotune"] = {"choices": tuned, "seconds": time.perf_counter() - t0,
                        "candidates": "FP64: MFMA tiles (waves, j-steps per LDS stage) (4,2), (4,4), (8,2); "
                                      "split: resident-panel grouped kernel with 8 or 4 waves vs streaming-panel grouped kernel"}

@@ -322,7 +322,8 @@ int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
  * reference's launch shape is fixed: 32 x 16 threads, cuda/cuda_solve.cu:98-99).
  *   rsh_batch_autotune    times every candidate configuration of the batch
  *                         class of (form, m) on a seeded problem of n_cells
- *                         cells, `iterations` graph-replayed iterations each,
+ *                         cells, `iterations` graph-replayed iterations each
+ *                         in three rounds (the fastest round of each counts),
  *                         and records the fastest for the device: FP64 classes
  *                         (row blocks per pass) choose the MFMA tile (waves,
  *                         j-steps per LDS stage) among (4,2), (4,4), (8,2);
