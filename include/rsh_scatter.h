@@ -572,6 +572,42 @@ int rsh_sweep_prepare_direction(int64_t n, const double* vertices, const int32_t
                                 int32_t* order, int32_t* face_state, double* x0_s, uint8_t* x0_face, int64_t cap,
                                 int64_t* n_records);
 
+/* The reference's MAKE step in its own numbering. Its meshes come from VTK:
+ * face f of a cell holds points {0,1,3}, {1,2,3}, {2,0,3}, {0,2,1} (f = 0..3),
+ * i.e. the face opposite point 2, 0, 1, 3. */
+
+/* The reference's input arrays from vertices [N][4][3] and pairs [N][4] in this
+ * library's numbering (face f opposite vertex f); each output may be NULL:
+ * vertex_matrix [N][4][4] (vertex.bin: Eigen Matrix4d, column j = (x, y, z, 1)
+ * of point j), faces [N][4][3][3] (grid.bin: the face's points A, B, C in VTK
+ * order), outward unit normals [N][4][3] (normals.bin) and pairs_vtk [N][4]
+ * (pairs.bin, face ids renumbered, boundary types kept). Synthetic-mesh helper:
+ * the reference writes these from its VTK reader (WriteCellFaces, WriteVertex,
+ * make_illum_struct/struct_short_characteristics_calculations.cpp:9-56). */
+int rsh_mesh_vtk_arrays(int64_t n, const double* vertices, const int32_t* pairs, double* vertex_matrix, double* faces,
+                        double* normals, int32_t* pairs_vtk);
+
+/* Face nodes [N][4][3][3] in the reference's placement and order (X.bin,
+ * BasePointTetra): node j of face f at weight 2/3 on one point of the face and
+ * 1/6 on the other two. Replaces MakeArrayX
+ * (make_illum_struct/struct_short_characteristics_logic_function.cpp:10-72). */
+int rsh_make_nodes(int64_t n, const double* vertex_matrix, double* nodes);
+
+/* One direction of the reference's MAKE loop (make_illum_struct/
+ * struct_short_characteristics_main.cpp:128-165 with GetNodes and
+ * CalculateNodeValue, struct_short_characteristics_logic_function.cpp:75-296):
+ * cells in the given order (graph<k>.bin), face_state [N] from each cell's own
+ * normals (bit f = in face, n_f . omega < -1e-10), and for every node of every
+ * out face the first in face whose plane the line along omega meets strictly
+ * inside its triangle: x0_s = distance to that point, x0_face = the face,
+ * x0_local [2] = the point in the face's local coordinates (0, 0) behind a
+ * boundary face (LocX0<k>.bin's cell_local records). RSH_E_ARG if order is not
+ * a permutation or a node finds no such face (where the reference stops with
+ * "Not found inner face"); RSH_E_NOMEM if cap is below *n_records. */
+int rsh_make_direction(int64_t n, const double* vertex_matrix, const double* faces, const double* normals,
+                       const int32_t* pairs, const int32_t* order, const double* omega, int32_t* face_state,
+                       double* x0_s, double* x0_local, uint8_t* x0_face, int64_t cap, int64_t* n_records);
+
 typedef struct rsh_sweep_s* rsh_sweep;
 
 /* Sweep object for n_cells cells and n_dirs directions on a device: pairs

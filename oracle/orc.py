@@ -1,4 +1,4 @@
-"""ctypes wrapper of the CPU oracle (oracle/scatter_oracle.cpp).
+"""ctypes wrapper of the CPU oracle (oracle/scatter_oracle.cpp, sweep_oracle.cpp, make_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never
@@ -34,6 +34,8 @@ _PROTOS = {
     "orc_moments": (None, [vp, i64, i32, i32, vp, vp, f64, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
     "orc_read_simple_bin": (i64, [C.c_char_p, i32, vp, i64]),
     "orc_sweep": (f64, [i64, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, i32]),
+    "orc_make_nodes": (None, [i64, vp, vp]),
+    "orc_make_direction": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(i64)]),
 }
 _cache = {}
 
@@ -208,3 +210,27 @@ def sweep(per_dir, pairs, scattering, illum_old, cls=1, nodes=None, threads=0, v
                          pairs.ctypes.data, None if nd is None else nd.ctypes.data, S.ctypes.data, cls,
                          illum.ctypes.data, threads)
     return illum, norm
+
+
+def make_nodes(vertex_matrix):
+    """MakeArrayX (oracle/make_oracle.cpp): face nodes [N,4,3,3] from vertex matrices [N,4,4]."""
+    vm = _c(vertex_matrix)
+    out = np.empty((vm.shape[0], 4, 3, 3))
+    load().orc_make_nodes(vm.shape[0], vm.ctypes.data, out.ctypes.data)
+    return out
+
+
+def make_direction(vertex_matrix, faces, normals, pairs, order, omega):
+    """The reference's MAKE loop for one direction: (status, face_state, s, in_face, x0 [R,2]);
+    status 1 = a node found no in face strictly inside its triangle."""
+    vm, fc, nr = _c(vertex_matrix), _c(faces), _c(normals)
+    pr, od, w = _c(pairs, np.int32), _c(order, np.int32), _c(omega)
+    n = vm.shape[0]
+    cap = 9 * n
+    st, s, x0, f = np.empty(n, np.int32), np.empty(cap), np.empty((cap, 2)), np.empty(cap, np.uint8)
+    cnt = C.c_int64()
+    rc = load().orc_make_direction(n, vm.ctypes.data, fc.ctypes.data, nr.ctypes.data, pr.ctypes.data, od.ctypes.data,
+                                   w.ctypes.data, st.ctypes.data, s.ctypes.data, x0.ctypes.data, f.ctypes.data, cap,
+                                   C.byref(cnt))
+    r = min(cnt.value, cap)
+    return rc, st, s[:r].copy(), f[:r].copy(), x0[:r].copy()

@@ -164,6 +164,9 @@ PROTOTYPES = {
     "rsh_mesh_cube_tetra": (C.c_int, [i32, vp, vp]),
     "rsh_tetra_geometry": (C.c_int, [i64, vp, vp, vp, vp, vp]),
     "rsh_sweep_prepare_direction": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(i64)]),
+    "rsh_mesh_vtk_arrays": (C.c_int, [i64, vp, vp, vp, vp, vp, vp]),
+    "rsh_make_nodes": (C.c_int, [i64, vp, vp]),
+    "rsh_make_direction": (C.c_int, [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, C.POINTER(i64)]),
     "rsh_sweep_create": (C.c_int, [i32, i64, i32, vp, vp, P_vp]),
     "rsh_sweep_destroy": (C.c_int, [vp]),
     "rsh_sweep_set_direction": (C.c_int, [vp, i32, vp, vp, vp, vp, i64]),

@@ -74,17 +74,129 @@ def prepare(vertices, pairs, dirs, threads: int = 16) -> List[Tuple[np.ndarray, 
         return list(ex.map(lambda w: prepare_direction(vertices, pairs, w), dirs))
 
 
+# ---------------------------------------------------------------- the reference's MAKE step
+
+def mesh_vtk_arrays(vertices, pairs):
+    """The reference's input arrays for a mesh given in this library's numbering (face f opposite vertex f),
+    in its VTK face numbering (face f = points (0,1,3), (1,2,3), (2,0,3), (0,2,1)): (vertex_matrix [N,4,4]
+    (vertex.bin; row j = point j as (x, y, z, 1), Eigen's column-major Matrix4d), faces [N,4,3,3] (grid.bin),
+    outward unit normals [N,4,3] (normals.bin), pairs [N,4] int32 (pairs.bin))."""
+    v = np.ascontiguousarray(vertices, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32)
+    n = v.shape[0]
+    vm, faces, normals, pv = np.empty((n, 4, 4)), np.empty((n, 4, 3, 3)), np.empty((n, 4, 3)), np.empty((n, 4), np.int32)
+    N.call("rsh_mesh_vtk_arrays", n, v.ctypes.data, pr.ctypes.data, vm.ctypes.data, faces.ctypes.data,
+           normals.ctypes.data, pv.ctypes.data)
+    return vm, faces, normals, pv
+
+
+def make_nodes(vertex_matrix) -> np.ndarray:
+    """Face nodes [N,4,3,3] in the reference's placement and order (X.bin; MakeArrayX)."""
+    vm = np.ascontiguousarray(vertex_matrix, np.float64)
+    nodes = np.empty((vm.shape[0], 4, 3, 3))
+    N.call("rsh_make_nodes", vm.shape[0], vm.ctypes.data, nodes.ctypes.data)
+    return nodes
+
+
+def make_direction(vertex_matrix, faces, normals, pairs, order, omega) -> Tuple[np.ndarray, ...]:
+    """One direction of the reference's MAKE loop: (face_state [N] int32, s [R], in_face [R] uint8, x0 [R,2])
+    for the cells in `order` (graph<k>.bin), in the reference's numbering."""
+    vm = np.ascontiguousarray(vertex_matrix, np.float64)
+    fc, nr = np.ascontiguousarray(faces, np.float64), np.ascontiguousarray(normals, np.float64)
+    pr, od = np.ascontiguousarray(pairs, np.int32), np.ascontiguousarray(order, np.int32)
+    w = np.ascontiguousarray(omega, np.float64)
+    n = vm.shape[0]
+    state = np.empty(n, np.int32)
+    cap = 9 * n
+    s, x0, face = np.empty(cap), np.empty((cap, 2)), np.empty(cap, np.uint8)
+    count = C.c_int64()
+    N.call("rsh_make_direction", n, vm.ctypes.data, fc.ctypes.data, nr.ctypes.data, pr.ctypes.data, od.ctypes.data,
+           w.ctypes.data, state.ctypes.data, s.ctypes.data, x0.ctypes.data, face.ctypes.data, cap, C.byref(count))
+    r = count.value
+    return state, s[:r].copy(), face[:r].copy(), x0[:r].copy()
+
+
+def make(vertex_matrix, faces, normals, pairs, orders, dirs, threads: int = 16):
+    """make_direction for every direction (row of dirs [M,3], with its order) on a thread pool:
+    a list of (order, face_state, s, in_face, x0)."""
+    dirs = np.asarray(dirs, np.float64).reshape(-1, 3)
+
+    def one(k):
+        st, s, f, x0 = make_direction(vertex_matrix, faces, normals, pairs, orders[k], dirs[k])
+        return np.asarray(orders[k], np.int32), st, s, f, x0
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(one, range(len(dirs))))
+
+
+def write_make_inputs(base_dir: str, vertex_matrix, faces, normals, pairs) -> None:
+    """vertex.bin, grid.bin, normals.bin and pairs.bin as the reference's MAKE step reads them
+    (ReadSimpleFileBin, ReadCellFaces, ReadNormalFile: an int32 cell count then the records)."""
+    from .scatter import write_simple_bin
+    n = len(vertex_matrix)
+    write_simple_bin(os.path.join(base_dir, "vertex.bin"), np.ascontiguousarray(vertex_matrix, np.float64), 16)
+    write_simple_bin(os.path.join(base_dir, "grid.bin"), np.ascontiguousarray(faces, np.float64).reshape(n, 36), 36)
+    write_simple_bin(os.path.join(base_dir, "normals.bin"), np.ascontiguousarray(normals, np.float64).reshape(n, 12),
+                     12)
+    write_simple_bin(os.path.join(base_dir, "pairs.bin"), np.ascontiguousarray(pairs, np.int32).reshape(-1))
+
+
+def run_make_module(base_dir: str, illum_geo: str, dirs, graph: str, a: int = 0, b: int = 0,
+                    threads: int = 16) -> int:
+    """The reference's RunMakeModule (make_illum_struct/struct_short_characteristics_main.cpp:14-177) on its
+    files: reads pairs.bin, grid.bin, vertex.bin, normals.bin from base_dir and graph<k>.bin (prefix `graph`),
+    writes state_face<k>.bin and LocX0<k>.bin for directions [a, b) (b = 0: all), X.bin and ResBound.bin
+    (empty: no inner-source faces are traced) under the prefix illum_geo. dirs: [M,3] unit vectors.
+    Returns the number of directions written."""
+    from .scatter import read_simple_bin, write_simple_bin
+    dirs = np.asarray(dirs, np.float64).reshape(-1, 3)
+    pairs = read_simple_bin(os.path.join(base_dir, "pairs.bin"), np.int32)
+    faces = read_simple_bin(os.path.join(base_dir, "grid.bin"), np.float64, 36)
+    vm = read_simple_bin(os.path.join(base_dir, "vertex.bin"), np.float64, 16)
+    normals = read_simple_bin(os.path.join(base_dir, "normals.bin"), np.float64, 12)
+    n = vm.shape[0]
+    if faces.shape[0] != n or normals.shape[0] != n or pairs.size != 4 * n:
+        raise ValueError("vertex.bin, grid.bin, normals.bin and pairs.bin disagree on the cell count")
+    b = len(dirs) if b == 0 else b
+    ks = list(range(a, b))
+    orders = {k: read_simple_bin(f"{graph}{k}.bin", np.int32) for k in ks}
+
+    def one(k):
+        st, s, f, x0 = make_direction(vm, faces, normals, pairs, orders[k], dirs[k])
+        write_simple_bin(f"{illum_geo}state_face{k}.bin", st)
+        _write_locx0(f"{illum_geo}LocX0{k}.bin", s, f, x0)
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(one, ks))
+    write_simple_bin(f"{illum_geo}X.bin", make_nodes(vm).reshape(n, 36), 36)
+    write_simple_bin(f"{illum_geo}ResBound.bin", np.empty(0))
+    return len(ks)
+
+
 # ---------------------------------------------------------------- files
 
-def write_direction_files(main_dir: str, k: int, order, face_state, s, in_face) -> None:
-    """graph<k>.bin, state_face<k>.bin (int32) and LocX0<k>.bin (48-byte cell_local records,
-    x0 left zero: the sweep does not read it) in simple-bin format."""
+def _write_locx0(path: str, s, in_face, x0=None) -> None:
+    from .scatter import write_simple_bin
+    rec = np.zeros(len(s), CELL_LOCAL)
+    rec["s"], rec["in_face"] = s, in_face
+    if x0 is not None:
+        rec["x0"] = x0
+    write_simple_bin(path, rec.view(np.uint8), CELL_LOCAL.itemsize)
+
+
+def read_locx0(path: str):
+    """LocX0<k>.bin: (s [R], in_face [R] uint8, x0 [R,2])."""
+    from .scatter import read_simple_bin
+    raw = read_simple_bin(path, np.uint8, CELL_LOCAL.itemsize)
+    rec = np.ascontiguousarray(raw).view(CELL_LOCAL).reshape(-1)
+    return np.ascontiguousarray(rec["s"]), np.ascontiguousarray(rec["in_face"]), np.ascontiguousarray(rec["x0"])
+
+
+def write_direction_files(main_dir: str, k: int, order, face_state, s, in_face, x0=None) -> None:
+    """graph<k>.bin, state_face<k>.bin (int32) and LocX0<k>.bin (48-byte cell_local records; x0, the local
+    point on the in face, zero when not given: the sweep does not read it) in simple-bin format."""
     from .scatter import write_simple_bin
     write_simple_bin(os.path.join(main_dir, f"graph{k}.bin"), np.ascontiguousarray(order, np.int32))
     write_simple_bin(os.path.join(main_dir, f"state_face{k}.bin"), np.ascontiguousarray(face_state, np.int32))
-    rec = np.zeros(len(s), CELL_LOCAL)
-    rec["s"], rec["in_face"] = s, in_face
-    write_simple_bin(os.path.join(main_dir, f"LocX0{k}.bin"), rec.view(np.uint8), CELL_LOCAL.itemsize)
+    _write_locx0(os.path.join(main_dir, f"LocX0{k}.bin"), s, in_face, x0)
 
 
 def read_direction_files(main_dir: str, k: int):
@@ -92,9 +204,8 @@ def read_direction_files(main_dir: str, k: int):
     from .scatter import read_simple_bin
     order = read_simple_bin(os.path.join(main_dir, f"graph{k}.bin"), np.int32)
     state = read_simple_bin(os.path.join(main_dir, f"state_face{k}.bin"), np.int32)
-    raw = read_simple_bin(os.path.join(main_dir, f"LocX0{k}.bin"), np.uint8, CELL_LOCAL.itemsize)
-    rec = np.ascontiguousarray(raw).view(CELL_LOCAL).reshape(-1)
-    return order, state, np.ascontiguousarray(rec["s"]), np.ascontiguousarray(rec["in_face"])
+    s, in_face, _ = read_locx0(os.path.join(main_dir, f"LocX0{k}.bin"))
+    return order, state, s, in_face
 
 
 # ---------------------------------------------------------------- GPU sweep

@@ -70,6 +70,36 @@ def test_sweep_matches_oracle(gpu, ne, m, cls):
     assert norm.max() == pytest.approx(ref_norm, rel=1e-10) if np.isfinite(ref_norm) else np.isinf(norm.max())
 
 
+@pytest.mark.parametrize("cls", [1, 0])
+def test_sweep_on_reference_placement_matches_oracle(gpu, cls):
+    """The sweep fed the records of the reference's own MAKE step (its VTK face numbering, node placement and
+    order: rsh_mesh_vtk_arrays, rsh_make_nodes, rsh_make_direction) against the oracle on the same records,
+    and, renumbered, against the sweep on this library's own records."""
+    v, pairs = sw.mesh_cube_tetra(5)
+    n = v.shape[0]
+    xyz, _, _ = rs.fibonacci_directions(51)
+    dirs = xyz[1:]  # direction 0 has y = 0, where the reference's in-face search stops
+    M = len(dirs)
+    own = sw.prepare(v, pairs, dirs)
+    vm, faces, normals, pv = sw.mesh_vtk_arrays(v, pairs)
+    ref_per = []
+    for k in range(M):
+        st, s, f, _ = sw.make_direction(vm, faces, normals, pv, own[k][0], dirs[k])
+        ref_per.append((own[k][0], st, s, f))
+    rng = np.random.default_rng(77 + cls)
+    S = rng.lognormal(size=(M, n))
+    old = rng.lognormal(size=(M, n, 4))
+    nodes_ref = sw.make_nodes(vm) if cls == 0 else None
+    ref, ref_norm = orc.sweep(ref_per, pv, S, old, cls=cls, nodes=nodes_ref)
+    assert ref_norm != -2.0
+    got, norm, levels = gpu_sweep(v, pv, ref_per, S, old, cls, nodes_ref)
+    check(got, ref)
+    assert norm.max() == pytest.approx(ref_norm, rel=1e-10)
+    mine, _, _ = gpu_sweep(v, pairs, own, S, old, cls, sw.tetra_geometry(v)[3] if cls == 0 else None)
+    back = got[:, :, [1, 2, 0, 3]]  # face g of this library's numbering is VTK face [1, 2, 0, 3][g]
+    assert float(np.max(np.abs(back - mine) / np.abs(mine))) <= TOL
+
+
 def test_sweep_order_and_files(gpu, tmp_path):
     """The same directions loaded from reference-format files (graph, state_face,
     LocX0) and from a reversed-within-level order give identical GPU results."""
