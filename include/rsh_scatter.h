@@ -298,7 +298,9 @@ int rsh_tuned_tile_f64(int32_t device, int32_t m, rsh_tile* tile);
  *                        (arrays written on other streams must be complete);
  *                        ds holds the directions and phase matrix; form selects
  *                        the FP64 state or a packed split state (RSH_FORM_BF16X2,
- *                        RSH_FORM_FP16X2, RSH_FORM_BF16X3; residual still FP64)
+ *                        RSH_FORM_FP16X2, RSH_FORM_BF16X3; residual still FP64;
+ *                        a split slot keeps B rounded to FP32, the update
+ *                        (1 - a) B + a S is formed in FP64)
  *   rsh_batch_run        enqueues `iterations` iterations of every active slot
  *                        on the batch's stream (asynchronous)
  *   rsh_batch_residuals  per slot: residual of its last iteration (-1 if none)

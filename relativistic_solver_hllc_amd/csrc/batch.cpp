@@ -64,9 +64,10 @@ struct Slot {
     double* I[2] = {nullptr, nullptr};  // FP64 form
     void* state[2] = {nullptr, nullptr}; // split forms
     float* cell_inv = nullptr;
-    double* B = nullptr;
-    size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I (M rows), state, cell_inv
-    size_t abytes = 0;                          // allocated size of B and (FP64) I: padded rows
+    double* B = nullptr;                        // FP64 form: source term
+    float* Bs = nullptr;                        // split forms: source term rounded to FP32
+    size_t fbytes = 0, sbytes = 0, cbytes = 0;  // sizes of B / I (M rows, FP64), state, cell_inv
+    size_t abytes = 0;                          // allocated size of B (Bs) and (FP64) I: padded rows
     F64Plan plan{};
     SplitPlan splan{};
     bool res = false;  // split form on the resident-panel grouped kernel
@@ -118,8 +119,9 @@ struct rsh_batch_s {
     char* arena = nullptr;
     int64_t arena_size = 0;
     std::map<int64_t, int64_t> free_list;  // offset -> size, coalesced
-    // Grow-only FP64 staging for seeded split-form slots (the FP64 intensities
-    // the state is packed from).
+    // Grow-only FP64 staging for seeded split-form slots: the FP64 intensities
+    // the state is packed from and the FP64 source term (the cell scale bounds
+    // both) before it is rounded into the slot's FP32 copy.
     double* staging = nullptr;
     int64_t staging_bytes = 0;
 };
@@ -175,10 +177,12 @@ void free_slot(rsh_batch b, Slot& s) {
     dev_free(b, s.state[1], s.sbytes);
     dev_free(b, s.cell_inv, s.cbytes);
     dev_free(b, s.B, s.abytes);
+    dev_free(b, s.Bs, s.abytes);
     s.I[0] = s.I[1] = nullptr;
     s.state[0] = s.state[1] = nullptr;
     s.cell_inv = nullptr;
     s.B = nullptr;
+    s.Bs = nullptr;
 }
 
 int slot_rb(const Slot& s) { return s.form == RSH_FORM_FP64 ? s.plan.rb_per_pass : s.splan.rb_per_pass; }
@@ -348,10 +352,11 @@ int64_t arena_round(int64_t bytes) { return (bytes + kArenaAlign - 1) / kArenaAl
 // rounded up to the arena alignment), so that a problem whose footprint fits the
 // budget fits an empty arena.
 int64_t footprint(int form, int64_t n, int m) {
-    const int64_t rows = arena_round(padded_rows(form, m) * n * 8);  // B (and FP64 I), padded rows
+    const int64_t rows = arena_round(padded_rows(form, m) * n * 8);  // FP64 B and I, padded rows
     if (form == RSH_FORM_FP64) return 3 * rows;                     // I ping-pong + B
-    if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)  // state ping-pong + B + scales
-        return 2 * arena_round(split_state_units(form, 1, n, m) * 16) + rows + arena_round(n * 4);
+    if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)  // state ping-pong + FP32 B + scales
+        return 2 * arena_round(split_state_units(form, 1, n, m) * 16) + arena_round(padded_rows(form, m) * n * 4) +
+               arena_round(n * 4);
     return -1;
 }
 
@@ -481,12 +486,22 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     const size_t fbytes = (size_t)ds->m * n_cells * sizeof(double);
     const size_t sbytes = form == RSH_FORM_FP64 ? 0 : (size_t)split_state_units(form, 1, n_cells, ds->m) * 16;
     s.fbytes = fbytes;
-    s.abytes = (size_t)padded_rows(form, ds->m) * n_cells * sizeof(double);
+    // The split forms keep the source term in FP32 (half the bytes the
+    // iteration epilogue reads; its rounding, 2^-24 relative, is far below
+    // their tolerance); the update (1 - a) B + a S is formed in FP64.
+    const size_t belem = form == RSH_FORM_FP64 ? sizeof(double) : sizeof(float);
+    const size_t bbytes = (size_t)ds->m * n_cells * belem;
+    s.abytes = (size_t)padded_rows(form, ds->m) * n_cells * belem;
     s.sbytes = sbytes;
     s.cbytes = form == RSH_FORM_FP64 ? 0 : (size_t)n_cells * sizeof(float);
-    hipError_t e = dev_alloc(b, s.abytes, (void**)&s.B);
-    if (e == hipSuccess && s.abytes > fbytes)  // zero padding rows of B (read by the iteration kernels)
-        e = hipMemsetAsync(reinterpret_cast<char*>(s.B) + fbytes, 0, s.abytes - fbytes, b->stream);
+    void* bsrc_alloc = nullptr;
+    hipError_t e = dev_alloc(b, s.abytes, &bsrc_alloc);
+    if (form == RSH_FORM_FP64)
+        s.B = static_cast<double*>(bsrc_alloc);
+    else
+        s.Bs = static_cast<float*>(bsrc_alloc);
+    if (e == hipSuccess && s.abytes > bbytes)  // zero padding rows of B (read by the iteration kernels)
+        e = hipMemsetAsync(static_cast<char*>(bsrc_alloc) + bbytes, 0, s.abytes - bbytes, b->stream);
     if (e == hipSuccess && form == RSH_FORM_FP64) {
         if ((e = dev_alloc(b, s.abytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, s.abytes, (void**)&s.I[1]);
         for (double* a : {s.I[0], s.I[1]})  // zero padding rows (read, and written back as zeros)
@@ -503,38 +518,43 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     }
     s.bytes = footprint(form, n_cells, ds->m);
     const int p = b->parity;  // the next iteration reads buffer p
+    const int64_t count = (int64_t)ds->m * n_cells;
     const double* src_I0 = d_I0;
+    const double* src_B = d_B;  // FP64 source term of a split slot (its cell scale, then rounded into Bs)
     if (seed) {
         const uint64_t sd = *seed;
-        e = launch_fill_uniform(s.B, (int64_t)ds->m * n_cells, 0.5, 2.0, sd * 2 + 2, b->stream);
-        if (e == hipSuccess && form == RSH_FORM_FP64) {
-            e = launch_fill_lognormal(s.I[p], (int64_t)ds->m * n_cells, 0.0, 1.0, sd * 2 + 1, b->stream);
-        } else if (e == hipSuccess) {
-            if ((int64_t)fbytes > b->staging_bytes) {
+        if (form == RSH_FORM_FP64) {
+            e = launch_fill_uniform(s.B, count, 0.5, 2.0, sd * 2 + 2, b->stream);
+            if (e == hipSuccess) e = launch_fill_lognormal(s.I[p], count, 0.0, 1.0, sd * 2 + 1, b->stream);
+        } else {
+            if (2 * (int64_t)fbytes > b->staging_bytes) {
                 // grow-only; the old staging may still be read by queued packs
                 RSH_HIP_TRY(hipStreamSynchronize(b->stream));
                 hipFree(b->staging);
                 b->staging = nullptr;
                 b->staging_bytes = 0;
-                if ((e = hipMalloc(&b->staging, fbytes)) == hipSuccess) b->staging_bytes = (int64_t)fbytes;
+                if ((e = hipMalloc(&b->staging, 2 * fbytes)) == hipSuccess) b->staging_bytes = 2 * (int64_t)fbytes;
             }
-            if (e == hipSuccess)
-                e = launch_fill_lognormal(b->staging, (int64_t)ds->m * n_cells, 0.0, 1.0, sd * 2 + 1, b->stream);
-            src_I0 = b->staging;
+            double* st_I0 = static_cast<double*>(b->staging);
+            double* st_B = st_I0 + count;
+            if (e == hipSuccess) e = launch_fill_uniform(st_B, count, 0.5, 2.0, sd * 2 + 2, b->stream);
+            if (e == hipSuccess) e = launch_fill_lognormal(st_I0, count, 0.0, 1.0, sd * 2 + 1, b->stream);
+            src_I0 = st_I0;
+            src_B = st_B;
         }
-    } else {
-        if ((e = hipMemcpyAsync(s.B, d_B, fbytes, hipMemcpyDeviceToDevice, b->stream)) == hipSuccess &&
-            form == RSH_FORM_FP64)
+    } else if (form == RSH_FORM_FP64) {
+        if ((e = hipMemcpyAsync(s.B, d_B, fbytes, hipMemcpyDeviceToDevice, b->stream)) == hipSuccess)
             e = hipMemcpyAsync(s.I[p], d_I0, fbytes, hipMemcpyDeviceToDevice, b->stream);
     }
     if (e == hipSuccess && form != RSH_FORM_FP64) {
         // padding positions of the other buffer are never written by the epilogue
         if ((e = hipMemsetAsync(s.state[p ^ 1], 0, sbytes, b->stream)) == hipSuccess) {
             if (pack_fused_supported(ds->m))  // scale (I0 and B bound every iterate) and pack in one pass
-                e = launch_pack_fused(form, src_I0, s.B, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
-            else if ((e = launch_cell_scale(src_I0, s.B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
+                e = launch_pack_fused(form, src_I0, src_B, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
+            else if ((e = launch_cell_scale(src_I0, src_B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
                 e = launch_pack_state(form, src_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
         }
+        if (e == hipSuccess) e = launch_f64_to_f32(src_B, s.Bs, count, b->stream);
     }
     if (e != hipSuccess) {
         free_slot(b, s);
@@ -555,6 +575,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     BatchProblem& P = b->h_probs[id];
     P = BatchProblem{};
     P.B = s.B;
+    P.Bs = s.Bs;
     P.resid = b->d_resid + 2 * id;
     P.n = n_cells;
     P.m = ds->m;

@@ -319,6 +319,21 @@ hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, u
     return hipGetLastError();
 }
 
+// FP64 -> FP32 copy (round to nearest): the split forms' source term.
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst,
+                                                         int64_t count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = (float)src[i];
+}
+
+hipError_t launch_f64_to_f32(const double* src, float* dst, int64_t count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, count);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Fill / scale utilities (kernels, so that they are safe inside captured graphs).
 // ---------------------------------------------------------------------------

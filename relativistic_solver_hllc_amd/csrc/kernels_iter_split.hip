@@ -64,7 +64,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
             const double oma = P.one_minus_albedo, alb = P.albedo;
             const int m = P.m;
             const int64_t n = P.n;
-            const __attribute__((address_space(1))) double* bsrc = (const __attribute__((address_space(1))) double*)P.B;
+            const __attribute__((address_space(1))) float* bsrc = (const __attribute__((address_space(1))) float*)P.Bs;
             const __attribute__((address_space(1))) float* rinv = (const __attribute__((address_space(1))) float*)P.row_inv;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 // row block's B loads go out together with no per-row branch; a
                 // padding row (k >= M) gets row scale 0, hence S = 0, nv = 0, parts
                 // 0 and a residual term 0 / 0 = NaN, which fmax drops.
-                double bv[4];
+                float bv[4];
                 float ri[4];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const double S = split_value(accm[rb][r], accx[rb][r]) * (double)ri[r] * cinv;
-                    const double nv = oma * bv[r] + alb * S;
+                    const double nv = oma * (double)bv[r] + alb * S;
                     float of = 0.0f;
 #pragma unroll
                     for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];

@@ -68,6 +68,14 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS
+// accesses, not for its global loads in flight (the state prefetch).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One K-step of MFMAs for TT tiles x RBW row blocks: A fragments of the parts
 // resident in LDS from a_base (this lane's vector of the step's first
 // fragment; NL parts per row block), the lo part of a three-part panel from
@@ -413,12 +421,19 @@ hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* s
 // one chunk run on one XCD at about the same time and share its L2 copy of the
 // state. Per item the row group's panel slice (all K-steps) and row scales go
 // into LDS, then the chunks are swept as in split_resident_kernel (state
-// prefetch across chunk boundaries, issued before the stores). The epilogue is
+// prefetch across chunk boundaries, issued before the stores).
+// KS (K-streamed panel, for long K ranges): the slice is not held whole; a
+// two-slot LDS ring holds one K-step each, the next K-step's part of the slice
+// is loaded into registers while the current one is multiplied, written into
+// the other slot after it and published by an LDS-only barrier per K-step. The
+// LDS then no longer bounds the row group: at M = 800 it holds 4 (three parts)
+// or 5 (two parts) row blocks instead of 2 or 3, so each state fragment read
+// from L2 feeds twice the MFMAs. The epilogue is
 // that of the streaming grouped kernel (kernels_iter_split.hip): S in FP64 from
 // the two accumulators and the exact scales, I_new = (1 - a) B + a S, the
 // residual against the state the MFMAs consumed, I_new re-split into the other
 // ping-pong buffer. Same products, same order: bit-identical to it.
-template <int RBW, int FMT, int NT, int STEPS, int W>
+template <int RBW, int FMT, int NT, int STEPS, int W, bool KS>
 __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, const int* __restrict__ meta,
     unsigned* __restrict__ counters, int parity) {
@@ -428,10 +443,14 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     static_assert(TT * W == kResChunkTiles, "waves must divide the 16 tiles of a chunk");
     constexpr int XD = (GXD > 0 && STEPS >= 25) ? GXD : (NT == 2 ? 3 : 2);  // shallower than the plain kernel: the epilogue needs the registers
     constexpr int steps = STEPS;
-    constexpr int64_t units = (int64_t)steps * RBW * NT * 64;
-    extern __shared__ v4u lds[];  // [steps][RBW][NT][64], then 16 RBW row scales
+    constexpr int64_t units = (int64_t)steps * RBW * NT * 64;  // panel slice of a row group
+    constexpr int SU = RBW * NT * 64;                          // one K-step of it
+    constexpr int64_t lds_units = KS ? 2 * SU : units;
+    // KS: 16-byte units of one K-step per thread (the last pass partial)
+    constexpr int PER = (SU + 64 * W - 1) / (64 * W);
+    extern __shared__ v4u lds[];  // [steps][RBW][NT][64] (KS: [2][RBW][NT][64]), then 16 RBW row scales
     __shared__ int s_item;
-    float* rscale = reinterpret_cast<float*>(lds + units);
+    float* rscale = reinterpret_cast<float*>(lds + lds_units);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -451,7 +470,33 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
         const int rg = item.pass;
         const int row0 = rg * RBW * 16;
         const v4u* __restrict__ src = static_cast<const v4u*>(P.spanel) + (int64_t)rg * units;
-        for (int64_t u = threadIdx.x; u < units; u += 64 * W) lds[u] = src[u];
+        // KS: the slice through a buffer descriptor (global loads, which an
+        // LDS-only barrier does not wait for) and K-step 0 into ring slot 0.
+        const __amdgpu_buffer_rsrc_t psrc =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(src), (short)0, (int)(units * 16), kBufWord3);
+        v4u preg[PER];
+        auto panel_load = [&](int step) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int u = (int)threadIdx.x + i * 64 * W;
+                if (SU % (64 * W) == 0 || u < SU)
+                    preg[i] = __builtin_bit_cast(
+                        v4u, __builtin_amdgcn_raw_buffer_load_b128(psrc, u * 16, step * SU * 16, 0));
+            }
+        };
+        auto panel_store = [&](int slot) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int u = (int)threadIdx.x + i * 64 * W;
+                if (SU % (64 * W) == 0 || u < SU) lds[slot * SU + u] = preg[i];
+            }
+        };
+        if constexpr (KS) {
+            panel_load(0);
+            panel_store(0);
+        } else {
+            for (int64_t u = threadIdx.x; u < units; u += 64 * W) lds[u] = src[u];
+        }
         for (int r = threadIdx.x; r < RBW * 16; r += 64 * W)
             rscale[r] = row0 + r < P.m ? P.row_inv[row0 + r] : 0.0f;
         __syncthreads();
@@ -515,10 +560,22 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
                     accm[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
                     accx[i][q] = v4f{0.f, 0.f, 0.f, 0.f};
                 }
+            // KS: K-step t of this chunk sits in ring slot (t + pc) & 1 (STEPS
+            // is odd, so the parity flips from chunk to chunk); the step after
+            // the last one is K-step 0 of the next chunk.
+            const int pc = (int)((chunk - c0) & 1);
+            const bool more_chunks = chunk + 1 < c1;
             static_for<0, STEPS>([&](auto tt) {
                 constexpr int t = decltype(tt)::value;
                 constexpr int sl = t % XD;
                 __builtin_amdgcn_sched_barrier(0);
+                // KS: the next K-step's panel load goes out before this step's
+                // state prefetch, so that waiting for it (before its LDS store)
+                // leaves the state loads in flight (one in-order counter).
+                const bool next = KS && (t + 1 < STEPS || more_chunks);
+                if constexpr (KS) {
+                    if (next) panel_load((t + 1) % STEPS);
+                }
                 V bx[TT][NT];
 #pragma unroll
                 for (int q = 0; q < TT; q++)
@@ -526,8 +583,16 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
                     for (int p = 0; p < NT; p++) bx[q][p] = __builtin_bit_cast(V, xr[sl][q][p]);
                 if constexpr (t + XD < STEPS) load_x(std::integral_constant<int, sl>{}, t + XD);
                 V ahi[RBW];
-                res_mfma_step<RBW, FMT, NT, TT, NT, false>(lds + (int64_t)t * RBW * NT * 64 + lane, nullptr, ahi,
-                                                                  bx, nullptr, accm, accx);
+                if constexpr (KS) {
+                    static_assert(STEPS % 2 == 1, "the ring parity assumes an odd K-step count");
+                    res_mfma_step<RBW, FMT, NT, TT, NT, false>(lds + ((t + pc) & 1) * SU + lane, nullptr, ahi, bx,
+                                                               nullptr, accm, accx);
+                    if (next) panel_store((t + 1 + pc) & 1);
+                    lds_barrier();
+                } else {
+                    res_mfma_step<RBW, FMT, NT, TT, NT, false>(lds + (int64_t)t * RBW * NT * 64 + lane, nullptr, ahi,
+                                                               bx, nullptr, accm, accx);
+                }
             });
             // This chunk's state (old values) and the output chunk through buffer
             // descriptors (uniform base, 32-bit offsets), kept before the next
@@ -559,10 +624,11 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
             // (tile q, row block i) pairs in order p = q * RBW + i; the loads of
             // pair p + LA are issued before the stores of pair p (the compiler
             // may not move a load above a store to another buffer: it cannot tell
-            // that they do not alias). LA: as far ahead as the registers allow.
+            // that they do not alias). LA = 2 measured best at 4 and 5 row
+            // blocks (with B in FP32; 4 was slower for three parts).
             constexpr int NPAIR = TT * RBW;
-            constexpr int LA = RBW <= 3 ? 3 : (RBW == 4 ? 2 : 1);
-            double bv[LA + 1][4];
+            constexpr int LA = RBW <= 3 ? 3 : 2;
+            float bv[LA + 1][4];
             v2u oldraw[LA + 1][NT];
             auto pair_rb = [&](int i, bool& rb_ok) {
                 const int rbg = rg * RBW + i;
@@ -578,13 +644,13 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
                 bool rb_ok;
                 const int rbe = pair_rb(i, rb_ok);
                 const int64_t cell = (ct0 + q) * 16 + cl;
-                const int cbyte = (int)((cell < n ? cell : n - 1) * 8);
+                const int cbyte = (int)((cell < n ? cell : n - 1) * 4);
                 const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<double*>(P.B) + (int64_t)rbe * 16 * n, (short)0, (int)(16 * n * 8), kBufWord3);
+                    const_cast<float*>(P.Bs) + (int64_t)rbe * 16 * n, (short)0, (int)(16 * n * 4), kBufWord3);
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    bv[buf][r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
-                                                                bsrc, (int)(4 * g * n * 8) + cbyte, (int)(r * n * 8), 0));
+                    bv[buf][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                               bsrc, (int)(4 * g * n * 4) + cbyte, (int)(r * n * 4), 0));
                 const int voff = pair_voff(q, rbe);
 #pragma unroll
                 for (int t = 0; t < NT; t++)
@@ -610,7 +676,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const double S = split_value(accm[i][q][r], accx[i][q][r]) * (double)rs[i * 16 + r] * ci;
-                    const double nv = oma * bv[buf][r] + alb * S;
+                    const double nv = oma * (double)bv[buf][r] + alb * S;
                     float of = 0.0f;
 #pragma unroll
                     for (int t = NT - 1; t >= 0; t--) {
@@ -642,6 +708,11 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     }
 }
 
+// The grouped iteration kernel streams the panel slice through LDS by K-step
+// (KS) at 25 K-steps (M = 800), where a whole slice would leave room for only
+// 2 or 3 row blocks; below that it holds the slice.
+constexpr bool res_k_streamed(int steps) { return steps >= 25; }
+
 // Largest row-block count per workgroup: the grouped iteration kernel (eight
 // waves, two per SIMD) keeps two parts in 256 registers up to five row blocks,
 // three parts up to four; the single-problem kernel (one wave per SIMD, 512
@@ -663,7 +734,8 @@ bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int*
     const int steps = (m + 31) / 32;
     const int rb_need = (rows + 15) / 16;
     const int lds_parts = grouped ? nt : kRes4LdsParts;
-    const int per_rb = steps * lds_parts * 1024 + 16 * 4;  // panel fragments + row scales
+    const int lds_steps = grouped && res_k_streamed(steps) ? 2 : steps;    // a two-slot ring of K-steps
+    const int per_rb = lds_steps * lds_parts * 1024 + 16 * 4;               // panel fragments + row scales
     int rbw_max = kResLdsBytes / per_rb;
     const int cap = grouped ? res_max_rbw(nt) : kRes4MaxRbw;
     if (rbw_max > cap) rbw_max = cap;
@@ -723,19 +795,21 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
 // 800). rb = RBW of the class, steps its K-steps; false if not specialised.
 bool iter_split_res_supported(int form, int rb, int steps) {
     const int nt = split_terms(form);
-    if (nt == 2) return (rb == 5 && (steps == 7 || steps == 13)) || (rb == 3 && steps == 25);
-    return (rb == 4 && (steps == 7 || steps == 13)) || (rb == 2 && steps == 25);
+    if (nt == 2) return rb == 5 && (steps == 7 || steps == 13 || steps == 25);
+    return rb == 4 && (steps == 7 || steps == 13 || steps == 25);
 }
 
 size_t iter_split_res_lds(int form, int rb, int steps) {
-    return (size_t)steps * rb * split_terms(form) * 1024 + (size_t)rb * 16 * 4;
+    const int lds_steps = res_k_streamed(steps) ? 2 : steps;
+    return (size_t)lds_steps * rb * split_terms(form) * 1024 + (size_t)rb * 16 * 4;
 }
 
 template <int RBW, int FMT, int NT, int STEPS>
 static hipError_t iter_res_one(bool launch, int waves, int grid, size_t lds, hipStream_t st, const BatchProblem* probs,
                                const BatchItem* items, const int* meta, unsigned* counters, int parity) {
-    auto kern = waves == 4 ? grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, 4>
-                           : grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, kResWaves>;
+    constexpr bool KS = res_k_streamed(STEPS);
+    auto kern = waves == 4 ? grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, 4, KS>
+                           : grouped_iter_split_res_kernel<RBW, FMT, NT, STEPS, kResWaves, KS>;
     if (!launch)
         return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds);
@@ -751,9 +825,9 @@ hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int step
     if (waves != 4 && waves != kResWaves) return hipErrorInvalidValue;
     const size_t lds = iter_split_res_lds(form, rb, steps);
 #define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, waves, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
-    RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(3, 0, 2, 25)
-    RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(3, 1, 2, 25)
-    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(2, 0, 3, 25)
+    RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(5, 0, 2, 25)
+    RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(5, 1, 2, 25)
+    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(4, 0, 3, 25)
 #undef RSH_IR
     return hipErrorInvalidValue;
 }

@@ -163,7 +163,7 @@ hipError_t launch_split_mfma(int form, const void* d_state, int faces, int64_t n
 struct BatchProblem {
     const double* panel;        // FP64 phase panel (faces = 1), pass-major
     double* I[2];               // ping-pong FP64 state [M][N]
-    const double* B;            // source term [M][N]
+    const double* B;            // FP64 form: source term [M][N]
     unsigned long long* resid;  // 2 parity slots, bit pattern of a non-negative double
     int64_t n;
     int m;
@@ -175,6 +175,7 @@ struct BatchProblem {
     const float* cell_inv;      // per-cell inverse scale of the state
     const void* spanel;         // split phase panel
     const float* row_inv;       // per-row inverse scale of the split panel
+    const float* Bs;            // source term [M][N] rounded to FP32 (whole row blocks, padding rows zero)
 };
 struct BatchItem {
     int problem;
@@ -215,6 +216,7 @@ hipError_t launch_moments_f64(const double* d_illum, int faces, int64_t n, int m
                               double* d_divimpuls, int faces_per_thread, hipStream_t st);
 
 hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st);
+hipError_t launch_f64_to_f32(const double* src, float* dst, int64_t count, hipStream_t st);
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
                                  hipStream_t st);
 

@@ -282,12 +282,13 @@ def test_arena_reuse_and_exhaustion(gpu):
 @pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
 def test_split_resident_kernel_equals_streaming(gpu, form):
     """The resident-panel grouped kernel (M = 200, 400, 800; 8 waves of 2 cell tiles
-    or 4 waves of 4) and the streaming-panel one compute the same products in the
-    same order: after several graph-replayed iterations of an uneven mix (ragged
-    cell counts, one M without a resident plan), states and residuals are
-    bit-identical."""
+    or 4 waves of 4; at M = 800 the panel streamed through LDS by K-step) and the
+    streaming-panel one compute the same products in the same order: after
+    several graph-replayed iterations of an uneven mix (ragged cell counts, one M
+    without a resident plan, an M = 800 problem of 20 cell chunks), states and
+    residuals are bit-identical."""
     cases = [(400, 2000, "hg", 0.6), (200, 1025, "thomson", 0.0), (800, 333, "hg", 0.9), (161, 257, "hg", 0.3),
-             (400, 17, "hg", 0.8), (200, 5000, "hg", 0.3)]
+             (400, 17, "hg", 0.8), (200, 5000, "hg", 0.3), (800, 5000, "thomson", 0.0)]
     probs = [make(m, n, ph, g, seed=90 + i) for i, (m, n, ph, g) in enumerate(cases)]
     out = {}
     for kind in ("streaming", "resident", "resident4", "auto"):
