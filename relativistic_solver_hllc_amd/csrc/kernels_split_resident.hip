@@ -56,9 +56,6 @@ constexpr int kBufWord3 = 0x00020000;
 template <int NT>
 constexpr int res_xd() { return NT == 2 ? 3 : 2; }
 constexpr int kResLoXd = 1;
-#ifndef GXD
-#define GXD 0
-#endif
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -441,7 +438,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     typedef typename Fmt<FMT>::T T;
     constexpr int TT = kResChunkTiles / W;  // 16-cell tiles per wave
     static_assert(TT * W == kResChunkTiles, "waves must divide the 16 tiles of a chunk");
-    constexpr int XD = (GXD > 0 && STEPS >= 25) ? GXD : (NT == 2 ? 3 : 2);  // shallower than the plain kernel: the epilogue needs the registers
+    constexpr int XD = NT == 2 ? 3 : 2;  // shallower than the plain kernel: the epilogue needs the registers
     constexpr int steps = STEPS;
     constexpr int64_t units = (int64_t)steps * RBW * NT * 64;  // panel slice of a row group
     constexpr int SU = RBW * NT * 64;                          // one K-step of it
@@ -452,7 +449,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
     __shared__ int s_item;
     float* rscale = reinterpret_cast<float*>(lds + lds_units);
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar tile offsets
     const int g = lane >> 4;
     const int cl = lane & 15;
     const int xcd = blockIdx.x % kXcds;
