@@ -57,8 +57,8 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                                                ct, 0, lds, accm, accx);
         RelMax rm;
         if (cell < P.n) {
-            const double cinv = (double)P.cell_inv[cell];
-            const double cscale = 1.0 / cinv;  // power of two: exact
+            const float cinvf = P.cell_inv[cell];
+            const float csf = 1.0f / cinvf;  // power of two: exact
             const unsigned char* in_bytes = reinterpret_cast<const unsigned char*>(st_in);
             // descriptor fields in registers across the stores; global-space source
             const double oma = P.one_minus_albedo, alb = P.albedo;
@@ -96,15 +96,18 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 }
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const double S = split_value(accm[rb][r], accx[rb][r]) * (double)ri[r] * cinv;
-                    const double nv = oma * (double)bv[r] + alb * S;
+                    // exact power-of-two scaling in FP32 and an explicit FMA, as
+                    // in the resident-panel kernel (bit-identical results)
+                    const float sc = ri[r] * cinvf;
+                    const double S = split_value(accm[rb][r] * sc, accx[rb][r] * sc);
+                    const double nv = __builtin_fma(alb, S, oma * (double)bv[r]);
                     float of = 0.0f;
 #pragma unroll
                     for (int t = NT - 1; t >= 0; t--) of += (float)oldp[t][r];
-                    const double old = (double)of * cinv;
+                    const double old = (double)(of * cinvf);
                     rm.add(old - nv, nv, true);
                     T q[NT];
-                    split_parts<FMT, NT>((float)(nv * cscale), q);
+                    split_parts<FMT, NT>((float)nv * csf, q);
 #pragma unroll
                     for (int t = 0; t < NT; t++) newp[t][r] = q[t];
                 }

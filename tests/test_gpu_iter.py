@@ -211,7 +211,7 @@ def test_add_is_ordered_after_null_stream_work(gpu, form):
         b.state(s, out)
         got = out.gather_columns(cells)
         ref, _ = orc.source_iteration(I0.gather_columns(cells), B.gather_columns(cells), P, 0.5, iters)
-        assert rel(got, ref) < (1e-12 if form == "fp64" else 2e-6)
+        assert rel(got, ref) < (1e-12 if form == "fp64" else 1e-6)
         for a in (I0, B, out):
             a.free()
 
