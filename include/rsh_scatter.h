@@ -315,8 +315,9 @@ int rsh_batch_destroy(rsh_batch b);
  * 8 waves where its plan fits one XCD and is specialised: M = 200, 400, 800
  * with n_cells < 2^24; else the streaming-panel one), 1 = resident panel,
  * 8 waves of 2 cell tiles (rsh_batch_add fails where it does not apply),
- * 2 = streaming panel, 3 = resident panel, 4 waves of 4 cell tiles (fails as
- * 1). All give bit-identical states and residuals. No reference counterpart
+ * 2 = streaming panel, 3 = resident panel, 4 waves of 2 cell tiles, two
+ * workgroups per CU where the LDS allows (fails as 1). All give bit-identical
+ * states and residuals. No reference counterpart
  * (batching is an extension). */
 int rsh_batch_set_split_kernel(rsh_batch b, int32_t kind);
 

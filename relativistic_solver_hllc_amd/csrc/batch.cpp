@@ -237,7 +237,9 @@ int rebuild(rsh_batch b) {
                 cs.res = true;
                 cs.steps = s.splan.steps;
                 cs.tile = rsh_tile{s.res_waves, 1};
-                cs.grid = device_cus(b->device) / kXcdLists * kXcdLists;  // one workgroup per CU (LDS-bound)
+                // one workgroup per CU (8 waves), or two (4 waves; they co-reside
+                // where the panel is streamed by K-step and the LDS is small)
+                cs.grid = device_cus(b->device) / kXcdLists * kXcdLists * (s.res_waves == 4 ? 2 : 1);
                 RSH_HIP_TRY(launch_grouped_iter_split_res(false, cs.form, rb, cs.steps, cs.tile.waves, 0, nullptr,
                                                           nullptr, nullptr, nullptr, 0, nullptr));
                 RSH_HIP_TRY(hipMalloc(&cs.d_meta, 2 * kXcdLists * sizeof(int)));

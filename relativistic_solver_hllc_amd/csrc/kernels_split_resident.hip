@@ -431,13 +431,18 @@ hipError_t launch_res(int blocks, size_t lds_bytes, hipStream_t st, const v4u* s
 // residual against the state the MFMAs consumed, I_new re-split into the other
 // ping-pong buffer. Same products, same order: bit-identical to it.
 template <int RBW, int FMT, int NT, int STEPS, int W, bool KS>
-__global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
+__global__ __launch_bounds__(64 * W, W == 4 ? 2 : 1) void grouped_iter_split_res_kernel(
     const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, const int* __restrict__ meta,
     unsigned* __restrict__ counters, int parity) {
     typedef typename Fmt<FMT>::V V;
     typedef typename Fmt<FMT>::T T;
-    constexpr int TT = kResChunkTiles / W;  // 16-cell tiles per wave
-    static_assert(TT * W == kResChunkTiles, "waves must divide the 16 tiles of a chunk");
+    // Chunk of CT 16-cell tiles, TT per wave: 8 waves x 2 tiles (one workgroup
+    // per CU), or 4 waves x 2 tiles (two workgroups per CU, whose epilogues and
+    // barriers are independent, so one's epilogue overlaps the other's MFMAs).
+    constexpr int TT = kResTiles;
+    constexpr int CT = W * TT;
+    constexpr int SUB = kIterResSub * kResChunkTiles / CT;  // chunks per item: an item is 1024 cells
+    static_assert(SUB * CT == kIterResSub * kResChunkTiles, "items of 1024 cells");
     constexpr int XD = NT == 2 ? 3 : 2;  // shallower than the plain kernel: the epilogue needs the registers
     constexpr int steps = STEPS;
     constexpr int64_t units = (int64_t)steps * RBW * NT * 64;  // panel slice of a row group
@@ -502,16 +507,16 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
         v4u* __restrict__ st_out = static_cast<v4u*>(P.state[parity ^ 1]);
         const int64_t n = P.n;
         const int64_t tiles = (n + 15) / 16;
-        const int64_t chunks = (tiles + kResChunkTiles - 1) / kResChunkTiles;
-        const int64_t c0 = item.tile * kIterResSub;
-        const int64_t c1 = c0 + kIterResSub < chunks ? c0 + kIterResSub : chunks;
+        const int64_t chunks = (tiles + CT - 1) / CT;
+        const int64_t c0 = item.tile * SUB;
+        const int64_t c1 = c0 + SUB < chunks ? c0 + SUB : chunks;
 
         __amdgpu_buffer_rsrc_t xs;
         int xoff[TT];
         int64_t ntile_of_bound = 0;
         auto bind_chunk = [&](int64_t chunk) {
-            const int64_t ct_chunk = chunk * kResChunkTiles;
-            const int ntile = (int)(tiles - ct_chunk < kResChunkTiles ? tiles - ct_chunk : kResChunkTiles);
+            const int64_t ct_chunk = chunk * CT;
+            const int ntile = (int)(tiles - ct_chunk < CT ? tiles - ct_chunk : CT);
             ntile_of_bound = ntile;
             xs = __builtin_amdgcn_make_buffer_rsrc(const_cast<v4u*>(st_in + ct_chunk * tile_units), (short)0,
                                                    ntile * tile_bytes, kBufWord3);
@@ -542,7 +547,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
             prefetch_head();
         }
         for (int64_t chunk = c0; chunk < c1; chunk++) {
-            const int64_t ct0 = chunk * kResChunkTiles + wave * TT;
+            const int64_t ct0 = chunk * CT + wave * TT;
             float cinv[TT];
 #pragma unroll
             for (int q = 0; q < TT; q++) {
@@ -597,7 +602,7 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_res_kernel(
             const __amdgpu_buffer_rsrc_t xin = xs;
             const int ntile_cur = (int)ntile_of_bound;
             const __amdgpu_buffer_rsrc_t xout = __builtin_amdgcn_make_buffer_rsrc(
-                st_out + chunk * kResChunkTiles * tile_units, (short)0, ntile_cur * tile_bytes, kBufWord3);
+                st_out + chunk * CT * tile_units, (short)0, ntile_cur * tile_bytes, kBufWord3);
             __builtin_amdgcn_sched_barrier(0);
             if (chunk + 1 < c1) {
                 bind_chunk(chunk + 1);

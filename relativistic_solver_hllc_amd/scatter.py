@@ -306,7 +306,8 @@ class Batch:
 
     def set_split_kernel(self, kind: str) -> None:
         """Kernel for split-form slots added afterwards: "auto", "resident" (resident panel, 8 waves of 2 cell
-        tiles), "resident4" (resident panel, 4 waves of 4 cell tiles) or "streaming"."""
+        tiles), "resident4" (resident panel, 4 waves of 2 cell tiles, two workgroups per CU where the LDS allows)
+        or "streaming"."""
         N.call("rsh_batch_set_split_kernel", self.handle,
                {"auto": 0, "resident": 1, "streaming": 2, "resident4": 3}[kind])
 

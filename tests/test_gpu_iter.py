@@ -282,7 +282,7 @@ def test_arena_reuse_and_exhaustion(gpu):
 @pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
 def test_split_resident_kernel_equals_streaming(gpu, form):
     """The resident-panel grouped kernel (M = 200, 400, 800; 8 waves of 2 cell tiles
-    or 4 waves of 4; at M = 800 the panel streamed through LDS by K-step) and the
+    or 4 waves of 2, two workgroups per CU; at M = 800 the panel streamed through LDS by K-step) and the
     streaming-panel one compute the same products in the same order: after
     several graph-replayed iterations of an uneven mix (ragged cell counts, one M
     without a resident plan, an M = 800 problem of 20 cell chunks), states and
