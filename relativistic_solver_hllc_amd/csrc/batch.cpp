@@ -357,8 +357,8 @@ int64_t footprint(int form, int64_t n, int m) {
     const int64_t rows = arena_round(padded_rows(form, m) * n * 8);  // FP64 B and I, padded rows
     if (form == RSH_FORM_FP64) return 3 * rows;                     // I ping-pong + B
     if (form == RSH_FORM_BF16X2 || form == RSH_FORM_FP16X2 || form == RSH_FORM_BF16X3)  // state ping-pong + FP32 B + scales
-        return 2 * arena_round(split_state_units(form, 1, n, m) * 16) + arena_round(padded_rows(form, m) * n * 4) +
-               arena_round(n * 4);
+        return 2 * arena_round(split_state_units(form, 1, n, m) * 16) +
+               arena_round(padded_rows(form, m) * ((n + 15) / 16 * 16) * 4) + arena_round(n * 4);
     return -1;
 }
 
@@ -491,9 +491,10 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
     // The split forms keep the source term in FP32 (half the bytes the
     // iteration epilogue reads; its rounding, 2^-24 relative, is far below
     // their tolerance); the update (1 - a) B + a S is formed in FP64.
-    const size_t belem = form == RSH_FORM_FP64 ? sizeof(double) : sizeof(float);
-    const size_t bbytes = (size_t)ds->m * n_cells * belem;
-    s.abytes = (size_t)padded_rows(form, ds->m) * n_cells * belem;
+    const size_t bbytes = (size_t)ds->m * n_cells * sizeof(double);
+    // split: FP32 in whole 16-cell tiles (MFMA output-tile order, padding written 0 by the conversion)
+    s.abytes = form == RSH_FORM_FP64 ? (size_t)padded_rows(form, ds->m) * n_cells * sizeof(double)
+                                     : (size_t)padded_rows(form, ds->m) * ((n_cells + 15) / 16 * 16) * sizeof(float);
     s.sbytes = sbytes;
     s.cbytes = form == RSH_FORM_FP64 ? 0 : (size_t)n_cells * sizeof(float);
     void* bsrc_alloc = nullptr;
@@ -502,7 +503,7 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
         s.B = static_cast<double*>(bsrc_alloc);
     else
         s.Bs = static_cast<float*>(bsrc_alloc);
-    if (e == hipSuccess && s.abytes > bbytes)  // zero padding rows of B (read by the iteration kernels)
+    if (e == hipSuccess && form == RSH_FORM_FP64 && s.abytes > bbytes)  // zero padding rows of B (read by the iteration kernels)
         e = hipMemsetAsync(static_cast<char*>(bsrc_alloc) + bbytes, 0, s.abytes - bbytes, b->stream);
     if (e == hipSuccess && form == RSH_FORM_FP64) {
         if ((e = dev_alloc(b, s.abytes, (void**)&s.I[0])) == hipSuccess) e = dev_alloc(b, s.abytes, (void**)&s.I[1]);
@@ -556,7 +557,8 @@ int add_impl(rsh_batch b, rsh_dirset ds, int64_t n_cells, double albedo, int for
             else if ((e = launch_cell_scale(src_I0, src_B, n_cells, ds->m, 1, s.cell_inv, b->stream)) == hipSuccess)
                 e = launch_pack_state(form, src_I0, 1, n_cells, ds->m, s.cell_inv, s.state[p], b->stream);
         }
-        if (e == hipSuccess) e = launch_f64_to_f32(src_B, s.Bs, count, b->stream);
+        if (e == hipSuccess)
+            e = launch_source_tiles(src_B, ds->m, n_cells, (int)padded_rows(form, ds->m), s.Bs, b->stream);
     }
     if (e != hipSuccess) {
         free_slot(b, s);

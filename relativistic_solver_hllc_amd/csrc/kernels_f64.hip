@@ -319,18 +319,38 @@ hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, u
     return hipGetLastError();
 }
 
-// FP64 -> FP32 copy (round to nearest): the split forms' source term.
-__global__ __launch_bounds__(256) void f64_to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst,
-                                                         int64_t count) {
+typedef float v4f_src __attribute__((ext_vector_type(4)));
+
+// The split forms' source term: FP64 [m][n] rounded to FP32 (nearest) in the
+// layout of the MFMA output tiles, so that an epilogue lane reads its four rows
+// of one cell as one 16-byte vector: element (k, c) at
+// ((k / 16 * tiles + c / 16) * 64 + 16 ((k % 16) / 4) + c % 16) * 4 + k % 4,
+// tiles = ceil(n / 16); rows up to rows_alloc (whole row blocks), cells up to
+// 16 tiles; positions past m or n hold 0. Thread = one 16-byte vector.
+__global__ __launch_bounds__(256) void source_tiles_kernel(const double* __restrict__ src, int m, int64_t n,
+                                                           int64_t tiles, int64_t vectors, v4f_src* __restrict__ dst) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = (float)src[i];
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < vectors; v += stride) {
+        const int lane = (int)(v & 63);
+        const int64_t rest = v >> 6;
+        const int64_t tile = rest % tiles, rb = rest / tiles;
+        const int64_t c = tile * 16 + (lane & 15);
+        const int k0 = (int)(rb * 16 + 4 * (lane >> 4));
+        v4f_src o;
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[r] = (k0 + r < m && c < n) ? (float)src[(int64_t)(k0 + r) * n + c] : 0.0f;
+        dst[v] = o;
+    }
 }
 
-hipError_t launch_f64_to_f32(const double* src, float* dst, int64_t count, hipStream_t st) {
-    if (count <= 0) return hipSuccess;
-    int64_t blocks = (count + 255) / 256;
+hipError_t launch_source_tiles(const double* src, int m, int64_t n, int rows_alloc, float* dst, hipStream_t st) {
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t vectors = (int64_t)(rows_alloc / 16) * tiles * 64;
+    if (vectors <= 0) return hipSuccess;
+    int64_t blocks = (vectors + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, count);
+    hipLaunchKernelGGL(source_tiles_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, m, n, tiles, vectors,
+                       reinterpret_cast<v4f_src*>(dst));
     return hipGetLastError();
 }
 

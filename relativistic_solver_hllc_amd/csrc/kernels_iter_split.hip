@@ -64,7 +64,9 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
             const double oma = P.one_minus_albedo, alb = P.albedo;
             const int m = P.m;
             const int64_t n = P.n;
-            const __attribute__((address_space(1))) float* bsrc = (const __attribute__((address_space(1))) float*)P.Bs;
+            const int64_t tiles = (n + 15) / 16;  // the source term is in output-tile order
+            typedef const __attribute__((address_space(1))) v4f* gcv4f_p;
+            const gcv4f_p bsrc = (gcv4f_p)P.Bs;
             const __attribute__((address_space(1))) float* rinv = (const __attribute__((address_space(1))) float*)P.row_inv;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
@@ -76,12 +78,11 @@ __global__ __launch_bounds__(64 * W) void grouped_iter_split_kernel(const BatchP
                 // row block's B loads go out together with no per-row branch; a
                 // padding row (k >= M) gets row scale 0, hence S = 0, nv = 0, parts
                 // 0 and a residual term 0 / 0 = NaN, which fmax drops.
-                float bv[4];
+                const v4f bv = bsrc[((int64_t)rbg * tiles + ct) * 64 + lane];  // rows 4g .. 4g + 3 of the cell
                 float ri[4];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int k = rbg * 16 + 4 * g + r;
-                    bv[r] = bsrc[(int64_t)k * n + cell];
                     ri[r] = k < m ? rinv[k] : 0.0f;
                 }
                 T oldp[NT][4], newp[NT][4];

@@ -645,14 +645,15 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 2 : 1) void grouped_iter_split_res
                 constexpr int pi = decltype(pp)::value, q = pi / RBW, i = pi % RBW, buf = pi % (LA + 1);
                 bool rb_ok;
                 const int rbe = pair_rb(i, rb_ok);
-                const int64_t cell = (ct0 + q) * 16 + cl;
-                const int cbyte = (int)((cell < n ? cell : n - 1) * 4);
+                // the source term in output-tile order: this lane's 4 rows of its
+                // cell are one 16-byte vector (the tile clamped into the array)
+                const int64_t tq = ct0 + q < tiles ? ct0 + q : tiles - 1;
                 const __amdgpu_buffer_rsrc_t bsrc = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<float*>(P.Bs) + (int64_t)rbe * 16 * n, (short)0, (int)(16 * n * 4), kBufWord3);
+                    const_cast<float*>(P.Bs) + (int64_t)rbe * tiles * 256, (short)0, (int)(tiles * 1024), kBufWord3);
+                const v4f b4 = __builtin_bit_cast(
+                    v4f, __builtin_amdgcn_raw_buffer_load_b128(bsrc, (int)((tq * 64 + lane) * 16), 0, 0));
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    bv[buf][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                               bsrc, (int)(4 * g * n * 4) + cbyte, (int)(r * n * 4), 0));
+                for (int r = 0; r < 4; r++) bv[buf][r] = b4[r];
                 const int voff = pair_voff(q, rbe);
 #pragma unroll
                 for (int t = 0; t < NT; t++)

@@ -175,7 +175,7 @@ struct BatchProblem {
     const float* cell_inv;      // per-cell inverse scale of the state
     const void* spanel;         // split phase panel
     const float* row_inv;       // per-row inverse scale of the split panel
-    const float* Bs;            // source term [M][N] rounded to FP32 (whole row blocks, padding rows zero)
+    const float* Bs;            // source term rounded to FP32, MFMA output-tile order (launch_source_tiles)
 };
 struct BatchItem {
     int problem;
@@ -216,7 +216,9 @@ hipError_t launch_moments_f64(const double* d_illum, int faces, int64_t n, int m
                               double* d_divimpuls, int faces_per_thread, hipStream_t st);
 
 hipError_t launch_fill_uniform(double* d, int64_t count, double lo, double hi, uint64_t seed, hipStream_t st);
-hipError_t launch_f64_to_f32(const double* src, float* dst, int64_t count, hipStream_t st);
+// Source term of a split slot: FP32 in MFMA output-tile order (whole row blocks
+// of rows_alloc rows and 16-cell tiles, padding 0).
+hipError_t launch_source_tiles(const double* src, int m, int64_t n, int rows_alloc, float* dst, hipStream_t st);
 hipError_t launch_fill_lognormal(double* d, int64_t count, double mu, double sigma, uint64_t seed,
                                  hipStream_t st);
 

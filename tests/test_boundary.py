@@ -155,12 +155,12 @@ def test_batch_footprint_counts_padded_rows(m, rows):
 
 @pytest.mark.parametrize("form", ["bf16x2", "fp16x2", "bf16x3"])
 def test_split_footprint_pads_source_rows(form):
-    """Split slots hold B in FP32 padded to a multiple of 16 rows: M = 17 and M = 32
-    need the same memory (one 32-direction K-step of state, 32 rows of B), M = 16
-    one row block of B less."""
+    """Split slots hold B in FP32 in whole 16-row blocks and 16-cell tiles: M = 17
+    and M = 32 need the same memory (one 32-direction K-step of state, 32 rows of
+    B), M = 16 one row block of B less (16 rows x 1008 cells for N = 1000)."""
     n = 1000
     assert rs.Batch.footprint(form, n, 17) == rs.Batch.footprint(form, n, 32)
-    assert rs.Batch.footprint(form, n, 17) - rs.Batch.footprint(form, n, 16) == 16 * n * 4
+    assert rs.Batch.footprint(form, n, 17) - rs.Batch.footprint(form, n, 16) == 16 * 1008 * 4
 
 
 @pytest.mark.parametrize("form", ["fp64", "bf16x2", "fp16x2", "bf16x3"])
@@ -178,7 +178,7 @@ def test_footprint_is_what_the_arena_carves(form):
         nt = 3 if form == "bf16x3" else 2
         steps = (m + 31) // 32
         state = -(-n // 16) * steps * nt * 64 * 16    # packed state units of 16 bytes
-        want = 2 * r256(state) + r256(208 * n * 4) + r256(4 * n)   # FP32 B
+        want = 2 * r256(state) + r256(208 * (-(-n // 16) * 16) * 4) + r256(4 * n)   # FP32 B in 16-cell tiles
     assert rs.Batch.footprint(form, n, m) == want
 
 
