@@ -724,9 +724,12 @@ constexpr bool res_k_streamed(int steps) { return steps >= 25; }
 
 // Largest row-block count per workgroup: the grouped iteration kernel (eight
 // waves, two per SIMD) keeps two parts in 256 registers up to five row blocks,
-// three parts up to four; the single-problem kernel (one wave per SIMD, 512
-// registers) holds five.
-constexpr int res_max_rbw(int nt) { return nt == 2 ? 5 : 4; }
+// three parts up to four with the whole slice in LDS and five with the
+// K-step ring (the eight-wave form spills three dwords, outside the K loop;
+// five row blocks take the 50 row blocks of M = 800 in 10 row groups, four
+// would pad them to 52);
+// the single-problem kernel (one wave per SIMD, 512 registers) holds five.
+constexpr int res_max_rbw(int nt, int steps) { return nt == 2 || res_k_streamed(steps) ? 5 : 4; }
 constexpr int kRes4MaxRbw = 5;
 constexpr int kRes4LdsParts = 2;  // panel parts resident in LDS in the single-problem kernel
 
@@ -746,7 +749,7 @@ bool plan_split_resident(int form, int m, int rows, int cus, SplitPlan* pl, int*
     const int lds_steps = grouped && res_k_streamed(steps) ? 2 : steps;    // a two-slot ring of K-steps
     const int per_rb = lds_steps * lds_parts * 1024 + 16 * 4;               // panel fragments + row scales
     int rbw_max = kResLdsBytes / per_rb;
-    const int cap = grouped ? res_max_rbw(nt) : kRes4MaxRbw;
+    const int cap = grouped ? res_max_rbw(nt, steps) : kRes4MaxRbw;
     if (rbw_max > cap) rbw_max = cap;
     if (rbw_max < 1 || rb_need < 1) return false;
     const int groups = (rb_need + rbw_max - 1) / rbw_max;
@@ -805,7 +808,7 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
 bool iter_split_res_supported(int form, int rb, int steps) {
     const int nt = split_terms(form);
     if (nt == 2) return rb == 5 && (steps == 7 || steps == 13 || steps == 25);
-    return rb == 4 && (steps == 7 || steps == 13 || steps == 25);
+    return (rb == 4 && (steps == 7 || steps == 13)) || (rb == 5 && steps == 25);
 }
 
 size_t iter_split_res_lds(int form, int rb, int steps) {
@@ -836,7 +839,7 @@ hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int step
 #define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, waves, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
     RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(5, 0, 2, 25)
     RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(5, 1, 2, 25)
-    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(4, 0, 3, 25)
+    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(5, 0, 3, 25)
 #undef RSH_IR
     return hipErrorInvalidValue;
 }
