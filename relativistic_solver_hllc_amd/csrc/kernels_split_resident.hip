@@ -723,13 +723,14 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 2 : 1) void grouped_iter_split_res
 constexpr bool res_k_streamed(int steps) { return steps >= 25; }
 
 // Largest row-block count per workgroup: the grouped iteration kernel (eight
-// waves, two per SIMD) keeps two parts in 256 registers up to five row blocks,
-// three parts up to four with the whole slice in LDS and five with the
-// K-step ring (the eight-wave form spills three dwords, outside the K loop;
-// five row blocks take the 50 row blocks of M = 800 in 10 row groups, four
-// would pad them to 52);
-// the single-problem kernel (one wave per SIMD, 512 registers) holds five.
-constexpr int res_max_rbw(int nt, int steps) { return nt == 2 || res_k_streamed(steps) ? 5 : 4; }
+// waves, two per SIMD) keeps two parts in 256 registers up to five row blocks;
+// three parts up to five at 7 K-steps (two dwords spill, in the prologue) and
+// with the K-step ring at 25 (the eight-wave form spills three dwords, outside
+// the K loop), four at 13, where a whole slice of five does not fit LDS. Five
+// row blocks take the 13 of M = 200 in 3 row groups and the 50 of M = 800 in
+// 10; four would pad them to 16 and 52. The single-problem kernel (one wave
+// per SIMD, 512 registers) holds five.
+constexpr int res_max_rbw(int nt, int steps) { return nt == 2 || res_k_streamed(steps) || steps <= 7 ? 5 : 4; }
 constexpr int kRes4MaxRbw = 5;
 constexpr int kRes4LdsParts = 2;  // panel parts resident in LDS in the single-problem kernel
 
@@ -808,7 +809,7 @@ hipError_t launch_split_resident(int form, const void* d_state, int64_t n, const
 bool iter_split_res_supported(int form, int rb, int steps) {
     const int nt = split_terms(form);
     if (nt == 2) return rb == 5 && (steps == 7 || steps == 13 || steps == 25);
-    return (rb == 4 && (steps == 7 || steps == 13)) || (rb == 5 && steps == 25);
+    return (rb == 5 && steps == 7) || (rb == 4 && steps == 13) || (rb == 5 && steps == 25);
 }
 
 size_t iter_split_res_lds(int form, int rb, int steps) {
@@ -839,7 +840,7 @@ hipError_t launch_grouped_iter_split_res(bool launch, int form, int rb, int step
 #define RSH_IR(RBW, F, T, S)                                                                                       if (fmt == F && nt == T && rb == RBW && steps == S)                                                                return iter_res_one<RBW, F, T, S>(launch, waves, grid, lds, st, d_probs, d_items, d_meta, d_counters, parity);
     RSH_IR(5, 0, 2, 7) RSH_IR(5, 0, 2, 13) RSH_IR(5, 0, 2, 25)
     RSH_IR(5, 1, 2, 7) RSH_IR(5, 1, 2, 13) RSH_IR(5, 1, 2, 25)
-    RSH_IR(4, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(5, 0, 3, 25)
+    RSH_IR(5, 0, 3, 7) RSH_IR(4, 0, 3, 13) RSH_IR(5, 0, 3, 25)
 #undef RSH_IR
     return hipErrorInvalidValue;
 }
