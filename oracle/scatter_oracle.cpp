@@ -33,7 +33,8 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp). The parity
 // build keeps -ffp-contract=off so the reference's sequential rounding order
-// is reproduced exactly; the baseline build (liborc_fast.so) uses -O3 / -Ofast.
+// is reproduced exactly; the baseline builds (liborc_o3.so, liborc_ofast.so)
+// use -O3 -march=x86-64-v3 and -Ofast -march=x86-64-v3.
 
 #include <cmath>
 #include <cstdint>
